@@ -1,0 +1,231 @@
+"""ctypes declarations of include/cfdart.h and include/cfdart_host.h."""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+EULER, ALLAIRE = 0, 1
+TRANSMISSIVE, REFLECTIVE = 0, 1
+MINMOD, VANLEER = 0, 1
+GARRICK, PVRS = 0, 1
+NAIVE, STRANG, ALTERNATING = 0, 1, 2
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+library_path = os.path.join(os.path.dirname(_HERE), "libcfdart.so")
+
+
+class CfdartError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"cfdart error {code}: {msg}")
+        self.code = code
+
+
+class Desc(C.Structure):
+    _fields_ = [
+        ("model", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("ng", C.c_int),
+        ("x0", C.c_double), ("y0", C.c_double), ("dx", C.c_double), ("dy", C.c_double),
+        ("gamma1", C.c_double), ("pinf1", C.c_double), ("gamma2", C.c_double), ("pinf2", C.c_double),
+        ("bc", C.c_int * 4), ("limiter", C.c_int), ("wave_speeds", C.c_int),
+        ("colsweep_uses_dx", C.c_int), ("z_lo", C.c_double), ("z_hi", C.c_double),
+    ]
+
+
+class DiscParams(C.Structure):
+    _fields_ = [
+        ("seed", C.c_ulonglong), ("n", C.c_int),
+        ("cx_lo", C.c_double), ("cx_hi", C.c_double), ("cy_lo", C.c_double), ("cy_hi", C.c_double),
+        ("r_lo", C.c_double), ("r_hi", C.c_double), ("p_lo", C.c_double), ("p_hi", C.c_double),
+        ("rho_lo", C.c_double), ("rho_hi", C.c_double), ("ambient", C.c_double * 4),
+    ]
+
+
+class BubbleParams(C.Structure):
+    _fields_ = [
+        ("seed", C.c_ulonglong), ("n", C.c_int),
+        ("cx_lo", C.c_double), ("cx_hi", C.c_double), ("cy_lo", C.c_double), ("cy_hi", C.c_double),
+        ("r_lo", C.c_double), ("r_hi", C.c_double),
+        ("water", C.c_double * 6), ("air", C.c_double * 6), ("postshock", C.c_double * 6),
+        ("shock_x", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load libcfdart.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(library_path):
+            raise CfdartError(-5, f"{library_path} not built (run __graft_entry__.build())")
+        L = C.CDLL(library_path)
+        P = C.POINTER
+        dp = P(C.c_double)
+        L.cfdart_last_error.restype = C.c_char_p
+        L.cfdart_device_count.restype = C.c_int
+        L.cfdart_create.argtypes = [P(Desc), P(C.c_void_p)]
+        L.cfdart_destroy.argtypes = [C.c_void_p]
+        L.cfdart_destroy.restype = None
+        L.cfdart_upload.argtypes = [C.c_void_p, dp]
+        L.cfdart_download.argtypes = [C.c_void_p, dp]
+        L.cfdart_sweep.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.cfdart_compute_dt.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, dp, dp]
+        L.cfdart_step.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double]
+        L.cfdart_run.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, dp, P(C.c_int)]
+        L.cfdart_set_time.argtypes = [C.c_void_p, C.c_double, C.c_int]
+        L.cfdart_stats.argtypes = [C.c_void_p, P(C.c_longlong)]
+        L.cfdart_sync.argtypes = [C.c_void_p]
+        L.cfdart_stream.argtypes = [C.c_void_p]
+        L.cfdart_stream.restype = C.c_void_p
+        L.cfdart_set_strip_rows.argtypes = [C.c_void_p, C.c_int]
+        L.cfdart_set_profiling.argtypes = [C.c_void_p, C.c_int]
+        L.cfdart_profile.argtypes = [C.c_void_p, dp, P(C.c_longlong)]
+        L.cfdart_ic_random_discs.argtypes = [P(Desc), P(DiscParams), dp]
+        L.cfdart_ic_shocked_bubbles.argtypes = [P(Desc), P(BubbleParams), dp]
+        L.cfdart_ic_planar.argtypes = [P(Desc), dp, dp, C.c_double, C.c_double, C.c_double, dp]
+        L.cfdart_prim_to_cons.argtypes = [P(Desc), dp, dp]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise CfdartError(rc, lib().cfdart_last_error().decode())
+    return rc
+
+
+def _dptr(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def device_count():
+    return lib().cfdart_device_count()
+
+
+def ncomp(model):
+    return 6 if model == ALLAIRE else 4
+
+
+def make_desc(model=EULER, nx=200, ny=200, x0=0.0, y0=0.0, lx=1.0, ly=1.0, gamma1=1.4, pinf1=0.0,
+              gamma2=1.4, pinf2=0.0, bc=(0, 0, 0, 0), limiter=MINMOD, wave_speeds=GARRICK,
+              colsweep_uses_dx=1, z_lo=0.0, z_hi=1.0):
+    d = Desc()
+    d.model, d.nx, d.ny, d.ng = model, nx, ny, 2
+    d.x0, d.y0, d.dx, d.dy = x0, y0, lx / nx, ly / ny
+    d.gamma1, d.pinf1, d.gamma2, d.pinf2 = gamma1, pinf1, gamma2, pinf2
+    for i in range(4):
+        d.bc[i] = bc[i]
+    d.limiter, d.wave_speeds, d.colsweep_uses_dx = limiter, wave_speeds, colsweep_uses_dx
+    d.z_lo, d.z_hi = z_lo, z_hi
+    return d
+
+
+def empty_state(d):
+    return np.zeros((d.ny + 2 * d.ng, d.nx + 2 * d.ng, ncomp(d.model)), dtype=np.float64)
+
+
+def random_discs(d, p):
+    h = empty_state(d)
+    _check(lib().cfdart_ic_random_discs(C.byref(d), C.byref(p), _dptr(h)))
+    return h
+
+
+def shocked_bubbles(d, p):
+    h = empty_state(d)
+    _check(lib().cfdart_ic_shocked_bubbles(C.byref(d), C.byref(p), _dptr(h)))
+    return h
+
+
+def planar(d, w_under, w_over, a, b, c):
+    h = empty_state(d)
+    wu = np.ascontiguousarray(w_under, dtype=np.float64)
+    wo = np.ascontiguousarray(w_over, dtype=np.float64)
+    _check(lib().cfdart_ic_planar(C.byref(d), _dptr(wu), _dptr(wo), a, b, c, _dptr(h)))
+    return h
+
+
+def prim_to_cons(d, w):
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    u = np.zeros(ncomp(d.model))
+    _check(lib().cfdart_prim_to_cons(C.byref(d), _dptr(w), _dptr(u)))
+    return u
+
+
+class Grid:
+    """An opaque device-resident grid (cfdart_grid*)."""
+
+    def __init__(self, desc):
+        self.desc = desc
+        h = C.c_void_p()
+        _check(lib().cfdart_create(C.byref(desc), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().cfdart_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, state):
+        state = np.ascontiguousarray(state, dtype=np.float64)
+        _check(lib().cfdart_upload(self.h, _dptr(state)))
+
+    def download(self, out=None):
+        if out is None:
+            out = empty_state(self.desc)
+        _check(lib().cfdart_download(self.h, _dptr(out)))
+        return out
+
+    def sweep(self, axis, dt):
+        _check(lib().cfdart_sweep(self.h, axis, dt))
+
+    def step(self, splitting, n, dt):
+        _check(lib().cfdart_step(self.h, splitting, n, dt))
+
+    def compute_dt(self, n, t, T, cfl):
+        dt = C.c_double()
+        mass = (C.c_double * 2)()
+        _check(lib().cfdart_compute_dt(self.h, n, t, T, cfl, C.byref(dt), mass))
+        return dt.value, (mass[0], mass[1])
+
+    def set_time(self, t, n):
+        _check(lib().cfdart_set_time(self.h, t, n))
+
+    def run(self, splitting, max_steps, T=math.inf, cfl=0.8, sync=True):
+        if not sync:
+            _check(lib().cfdart_run(self.h, splitting, max_steps, T, cfl, None, None))
+            return None
+        t = C.c_double()
+        n = C.c_int()
+        _check(lib().cfdart_run(self.h, splitting, max_steps, T, cfl, C.byref(t), C.byref(n)))
+        return t.value, n.value
+
+    def stats(self):
+        c = (C.c_longlong * 4)()
+        _check(lib().cfdart_stats(self.h, c))
+        return list(c)
+
+    def sync(self):
+        _check(lib().cfdart_sync(self.h))
+
+    def stream(self):
+        return lib().cfdart_stream(self.h)
+
+    def set_profiling(self, enable):
+        _check(lib().cfdart_set_profiling(self.h, 1 if enable else 0))
+
+    def profile(self):
+        ms = (C.c_double * 2)()
+        n = (C.c_longlong * 2)()
+        _check(lib().cfdart_profile(self.h, ms, n))
+        return {"x": (ms[0], n[0]), "y": (ms[1], n[1])}
+
+    def set_strip_rows(self, rows):
+        _check(lib().cfdart_set_strip_rows(self.h, rows))

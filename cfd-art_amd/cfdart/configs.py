@@ -1,0 +1,58 @@
+"""Named workloads: the benchmark's configurations as grid descriptors + initial conditions.
+
+euler_200      Euler, gamma 1.4, 200x200 on [0,1]^2, ambient (1,0,0,1), 8 seeded discs
+               (centres U[0.1,0.9]^2, r U[0.03,0.10], p U[5,20], rho U[0.5,2]), transmissive,
+               van Leer, HLLC, alternating splitting, CFL 0.8, run to t = 0.2.
+euler_2048     As above at 2048x2048 with 64 discs of r U[0.005,0.03]; reflective left/right,
+               transmissive bottom/top; 100 steps at CFL 0.8.
+allaire_1024   Allaire water (gamma 4.4, pinf 6e8, rho 1000) with 16 air bubbles (gamma 1.4,
+               rho 1), p 1e5, radii U[0.02,0.06], centres U[0.15,0.95]x[0.05,0.95]; a planar
+               shock at x = 0.1 (post-shock water p 1e9, rho 1323.65, u 681.58); z clipped to
+               [1e-6, 1-1e-6]; PVRS-HLLC; 2000 steps at CFL 0.5.
+"""
+from . import _lib as L
+
+Z_EPS = 1e-6
+
+
+def euler_discs(n=200, n_discs=8, r=(0.03, 0.10), bc=(0, 0, 0, 0), seed=1, limiter=L.VANLEER, ny=None):
+    ny = n if ny is None else ny
+    d = L.make_desc(L.EULER, n, ny, lx=1.0, ly=ny / n, gamma1=1.4, bc=bc, limiter=limiter)
+    p = L.DiscParams()
+    p.seed, p.n = seed, n_discs
+    p.cx_lo, p.cx_hi, p.cy_lo, p.cy_hi = 0.1, 0.9, 0.1 * ny / n, 0.9 * ny / n
+    p.r_lo, p.r_hi = r
+    p.p_lo, p.p_hi, p.rho_lo, p.rho_hi = 5.0, 20.0, 0.5, 2.0
+    for i, v in enumerate((1.0, 0.0, 0.0, 1.0)):
+        p.ambient[i] = v
+    return d, L.random_discs(d, p)
+
+
+def euler_200(seed=1):
+    return euler_discs(200, 8, (0.03, 0.10), (0, 0, 0, 0), seed)
+
+
+def euler_2048(seed=1):
+    return euler_discs(2048, 64, (0.005, 0.03), (1, 1, 0, 0), seed)
+
+
+def allaire_bubbles(n=1024, n_bubbles=16, seed=1, ny=None, wave_speeds=L.PVRS, limiter=L.MINMOD):
+    ny = n if ny is None else ny
+    d = L.make_desc(L.ALLAIRE, n, ny, lx=1.0, ly=ny / n, gamma1=4.4, pinf1=6e8, gamma2=1.4, pinf2=0.0,
+                    bc=(0, 0, 0, 0), limiter=limiter, wave_speeds=wave_speeds, z_lo=Z_EPS, z_hi=1.0 - Z_EPS)
+    zw, za = 1.0 - Z_EPS, Z_EPS
+    p = L.BubbleParams()
+    p.seed, p.n = seed, n_bubbles
+    p.cx_lo, p.cx_hi, p.cy_lo, p.cy_hi = 0.15, 0.95, 0.05 * ny / n, 0.95 * ny / n
+    p.r_lo, p.r_hi = 0.02, 0.06
+    water = (zw * 1000.0, (1 - zw) * 1.0, 0.0, 0.0, 1e5, zw)
+    air = (za * 1000.0, (1 - za) * 1.0, 0.0, 0.0, 1e5, za)
+    post = (zw * 1323.65, (1 - zw) * 1.0, 681.58, 0.0, 1e9, zw)
+    for i in range(6):
+        p.water[i], p.air[i], p.postshock[i] = water[i], air[i], post[i]
+    p.shock_x = 0.1
+    return d, L.shocked_bubbles(d, p)
+
+
+def allaire_1024(seed=1):
+    return allaire_bubbles(1024, 16, seed)

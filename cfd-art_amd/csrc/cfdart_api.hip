@@ -1,0 +1,444 @@
+// C-ABI of the sweep library (declarations and conventions in include/cfdart.h).
+#include "cfdart.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sweep_kernels.h"
+
+namespace cfd {
+hipError_t launch_sweep(const GridView& g, int model, int axis, bool last, double dt_arg,
+                        double dt_factor, int use_ctrl, int yH, hipStream_t st);
+hipError_t launch_reduce_dt(const GridView& g, int model, double* partials, hipStream_t st);
+hipError_t launch_init_dt(Ctrl* c, double dx, double dy, hipStream_t st);
+hipError_t launch_finalize(Ctrl* c, double dx, double dy, hipStream_t st);
+hipError_t launch_set_run(Ctrl* c, double T, double cfl, hipStream_t st);
+hipError_t launch_set_time(Ctrl* c, double t, int n, hipStream_t st);
+int reduce_blocks(const GridView& g);
+}  // namespace cfd
+
+struct cfdart_grid {
+    cfdart_desc d;
+    int nc;
+    int pitch;
+    long long plane;
+    double* buf[2];
+    int cur;
+    cfd::Ctrl* ctrl;
+    double* partials;
+    int nred;
+    hipStream_t st;
+    int yH;
+    // optional per-kernel timing: an event before and after every sweep launch
+    int profiling;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, int>> ev_marks;  // (kind, index of start event); end = start + 1
+    size_t ev_used;
+    double prof_ms[3];
+    long long prof_n[3];
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess) return fail(CFDART_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int check_desc(const cfdart_desc* d)
+{
+    if (!d) return fail(CFDART_E_ARG, "null descriptor");
+    if (d->model != CFDART_EULER && d->model != CFDART_ALLAIRE) return fail(CFDART_E_ARG, "model must be 0 (Euler) or 1 (Allaire)");
+    if (d->nx < 2 || d->ny < 2) return fail(CFDART_E_ARG, "nx and ny must be >= 2");
+    if (d->ng != 2) return fail(CFDART_E_ARG, "ng must be 2 (MUSCL-Hancock stencil)");
+    if (!(d->dx > 0.0) || !(d->dy > 0.0)) return fail(CFDART_E_ARG, "dx and dy must be positive");
+    if (!(d->gamma1 > 1.0) || (d->model == CFDART_ALLAIRE && !(d->gamma2 > 1.0)))
+        return fail(CFDART_E_ARG, "gamma must be > 1");
+    for (int i = 0; i < 4; ++i)
+        if (d->bc[i] != 0 && d->bc[i] != 1) return fail(CFDART_E_ARG, "bc must be 0 (transmissive) or 1 (reflective)");
+    if (d->limiter != 0 && d->limiter != 1) return fail(CFDART_E_ARG, "limiter must be 0 (minmod) or 1 (van Leer)");
+    if (d->wave_speeds != 0 && d->wave_speeds != 1) return fail(CFDART_E_ARG, "wave_speeds must be 0 or 1");
+    if (d->wave_speeds == 1 && d->model != CFDART_ALLAIRE) return fail(CFDART_E_ARG, "PVRS wave speeds exist for the Allaire model only");
+    if (d->model == CFDART_ALLAIRE && !(d->z_lo >= 0.0 && d->z_lo <= d->z_hi && d->z_hi <= 1.0))
+        return fail(CFDART_E_ARG, "need 0 <= z_lo <= z_hi <= 1");
+    if ((long long)d->nx * d->ny > (1LL << 31)) return fail(CFDART_E_ARG, "grid too large");
+    return 0;
+}
+
+cfd::GridView view(const cfdart_grid* g)
+{
+    cfd::GridView v;
+    v.cur = g->buf[g->cur];
+    v.nxt = g->buf[1 - g->cur];
+    v.plane = g->plane;
+    v.pitch = g->pitch;
+    v.nx = g->d.nx;
+    v.ny = g->d.ny;
+    for (int i = 0; i < 4; ++i) v.bc[i] = g->d.bc[i];
+    v.mat = cfd::make_material(g->d.gamma1, g->d.pinf1, g->d.gamma2, g->d.pinf2);
+    v.opt.limiter = g->d.limiter;
+    v.opt.wave_speeds = g->d.wave_speeds;
+    v.opt.z_lo = g->d.z_lo;
+    v.opt.z_hi = g->d.z_hi;
+    v.dx = g->d.dx;
+    v.dy = g->d.dy;
+    v.dx_half_col = g->d.colsweep_uses_dx ? g->d.dx : g->d.dy;
+    v.ctrl = g->ctrl;
+    return v;
+}
+
+hipEvent_t next_event(cfdart_grid* g)
+{
+    if (g->ev_used == g->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        g->ev_pool.push_back(e);
+    }
+    return g->ev_pool[g->ev_used++];
+}
+
+// Accumulates the recorded kernel times (needs the stream to be idle).
+void harvest_profile(cfdart_grid* g)
+{
+    for (auto& m : g->ev_marks) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, g->ev_pool[m.second], g->ev_pool[m.second + 1]) == hipSuccess) {
+            g->prof_ms[m.first] += ms;
+            g->prof_n[m.first] += 1;
+        }
+    }
+    g->ev_marks.clear();
+    g->ev_used = 0;
+}
+
+int sweep_once(cfdart_grid* g, int axis, bool last, double dt_arg, double factor, int use_ctrl)
+{
+    cfd::GridView v = view(g);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (g->profiling) {
+        if (g->ev_used + 2 > 4096) {
+            if (hipStreamSynchronize(g->st) != hipSuccess) return fail(CFDART_E_HIP, "sync failed");
+            harvest_profile(g);
+        }
+        size_t i0 = g->ev_used;
+        e0 = next_event(g);
+        e1 = next_event(g);
+        if (!e0 || !e1 || hipEventRecord(e0, g->st) != hipSuccess) return fail(CFDART_E_HIP, "event record failed");
+        g->ev_marks.push_back({axis, (int)i0});
+    }
+    hipError_t e = cfd::launch_sweep(v, g->d.model, axis, last, dt_arg, factor, use_ctrl, g->yH, g->st);
+    if (e != hipSuccess) return fail(CFDART_E_HIP, std::string("sweep launch: ") + hipGetErrorString(e));
+    if (g->profiling && hipEventRecord(e1, g->st) != hipSuccess) return fail(CFDART_E_HIP, "event record failed");
+    g->cur = 1 - g->cur;  // post_sweep: grid.swap(future_grid)
+    return 0;
+}
+
+// Sweep sequence of one step of the splitting scheme: axes and dt factors.
+int step_plan(int splitting, int n, int* axes, double* factors)
+{
+    if (splitting == CFDART_STRANG) {
+        axes[0] = 0; factors[0] = 0.5;
+        axes[1] = 1; factors[1] = 1.0;
+        axes[2] = 0; factors[2] = 0.5;
+        return 3;
+    }
+    if (splitting == CFDART_ALTERNATING && (n % 2) == 1) {
+        axes[0] = 1; factors[0] = 1.0;
+        axes[1] = 0; factors[1] = 1.0;
+        return 2;
+    }
+    axes[0] = 0; factors[0] = 1.0;
+    axes[1] = 1; factors[1] = 1.0;
+    return 2;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cfdart_last_error(void) { return g_err.c_str(); }
+
+int cfdart_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int cfdart_create(const cfdart_desc* desc, cfdart_grid** out)
+{
+    if (!out) return fail(CFDART_E_ARG, "null output pointer");
+    *out = nullptr;
+    int rc = check_desc(desc);
+    if (rc) return rc;
+    if (cfdart_device_count() < 1) return fail(CFDART_E_NODEV, "no HIP device");
+    cfdart_grid* g = new cfdart_grid();
+    g->d = *desc;
+    g->nc = desc->model == CFDART_ALLAIRE ? 6 : 4;
+    g->pitch = (desc->nx + 31) / 32 * 32;  // 256-byte aligned rows
+    g->plane = (long long)g->pitch * desc->ny;
+    g->cur = 0;
+    g->yH = 32;
+    g->buf[0] = g->buf[1] = nullptr;
+    g->ctrl = nullptr;
+    g->partials = nullptr;
+    g->profiling = 0;
+    g->ev_used = 0;
+    for (int i = 0; i < 3; ++i) { g->prof_ms[i] = 0.0; g->prof_n[i] = 0; }
+    size_t bytes = sizeof(double) * (size_t)g->plane * g->nc;
+    if (hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking) != hipSuccess) {
+        delete g;
+        return fail(CFDART_E_HIP, "hipStreamCreate failed");
+    }
+    if (hipMalloc(&g->buf[0], bytes) != hipSuccess || hipMalloc(&g->buf[1], bytes) != hipSuccess ||
+        hipMalloc(&g->ctrl, sizeof(cfd::Ctrl)) != hipSuccess) {
+        cfdart_destroy(g);
+        return fail(CFDART_E_NOMEM, "hipMalloc of the grid state failed");
+    }
+    cfd::GridView v = view(g);
+    g->nred = cfd::reduce_blocks(v);
+    if (hipMalloc(&g->partials, sizeof(double) * 2 * g->nred) != hipSuccess) {
+        cfdart_destroy(g);
+        return fail(CFDART_E_NOMEM, "hipMalloc of the reduction buffer failed");
+    }
+    (void)hipMemset(g->buf[0], 0, bytes);
+    (void)hipMemset(g->buf[1], 0, bytes);
+    (void)hipMemset(g->ctrl, 0, sizeof(cfd::Ctrl));
+    if (hipDeviceSynchronize() != hipSuccess) {
+        cfdart_destroy(g);
+        return fail(CFDART_E_HIP, "device initialisation failed");
+    }
+    *out = g;
+    return 0;
+}
+
+void cfdart_destroy(cfdart_grid* g)
+{
+    if (!g) return;
+    if (g->st) (void)hipStreamSynchronize(g->st);
+    if (g->buf[0]) (void)hipFree(g->buf[0]);
+    if (g->buf[1]) (void)hipFree(g->buf[1]);
+    if (g->ctrl) (void)hipFree(g->ctrl);
+    if (g->partials) (void)hipFree(g->partials);
+    for (hipEvent_t e : g->ev_pool) (void)hipEventDestroy(e);
+    if (g->st) (void)hipStreamDestroy(g->st);
+    delete g;
+}
+
+int cfdart_upload(cfdart_grid* g, const double* host)
+{
+    if (!g || !host) return fail(CFDART_E_ARG, "null argument");
+    const int nx = g->d.nx, ny = g->d.ny, ng = g->d.ng, nc = g->nc;
+    const int W = nx + 2 * ng;
+    std::vector<double> soa((size_t)g->plane * nc, 0.0);
+    for (int k = 0; k < nc; ++k)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x)
+                soa[(size_t)k * g->plane + (size_t)y * g->pitch + x] =
+                    host[((size_t)(y + ng) * W + (x + ng)) * nc + k];
+    HIPCHK(hipStreamSynchronize(g->st));
+    g->cur = 0;
+    HIPCHK(hipMemcpy(g->buf[0], soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(g->ctrl, 0, sizeof(cfd::Ctrl)));
+    HIPCHK(hipDeviceSynchronize());
+    return 0;
+}
+
+int cfdart_download(cfdart_grid* g, double* host)
+{
+    if (!g || !host) return fail(CFDART_E_ARG, "null argument");
+    const int nx = g->d.nx, ny = g->d.ny, ng = g->d.ng, nc = g->nc;
+    const int W = nx + 2 * ng;
+    std::vector<double> soa((size_t)g->plane * nc);
+    HIPCHK(hipStreamSynchronize(g->st));
+    HIPCHK(hipMemcpy(soa.data(), g->buf[g->cur], sizeof(double) * soa.size(), hipMemcpyDeviceToHost));
+    auto at = [&](int i, int j, int k) -> double& { return host[((size_t)i * W + j) * nc + k]; };
+    for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x)
+            for (int k = 0; k < nc; ++k) at(y + ng, x + ng, k) = soa[(size_t)k * g->plane + (size_t)y * g->pitch + x];
+    // ghost frame as pre_sweep fills it (E/sourcecode/euler_ppsweep.cpp:7-113)
+    const int mx = g->d.model == CFDART_ALLAIRE ? 2 : 1, my = mx + 1;
+    for (int i = ng; i < ny + ng; ++i)
+        for (int j = 0; j < ng; ++j)
+            for (int k = 0; k < nc; ++k) {
+                at(i, j, k) = at(i, 2 * ng - j - 1, k);
+                at(i, j + nx + ng, k) = at(i, nx + ng - 1 - j, k);
+            }
+    for (int j = ng; j < nx + ng; ++j)
+        for (int i = 0; i < ng; ++i)
+            for (int k = 0; k < nc; ++k) {
+                at(i, j, k) = at(2 * ng - i - 1, j, k);
+                at(i + ny + ng, j, k) = at(ny + ng - 1 - i, j, k);
+            }
+    for (int i = ng; i < ny + ng; ++i)
+        for (int j = 0; j < ng; ++j) {
+            if (g->d.bc[0]) at(i, j, mx) = -at(i, j, mx);
+            if (g->d.bc[1]) at(i, j + nx + ng, mx) = -at(i, j + nx + ng, mx);
+        }
+    for (int j = ng; j < nx + ng; ++j)
+        for (int i = 0; i < ng; ++i) {
+            if (g->d.bc[2]) at(i, j, my) = -at(i, j, my);
+            if (g->d.bc[3]) at(i + ny + ng, j, my) = -at(i + ny + ng, j, my);
+        }
+    return 0;
+}
+
+int cfdart_sweep(cfdart_grid* g, int axis, double dt)
+{
+    if (!g || (axis != 0 && axis != 1)) return fail(CFDART_E_ARG, "bad handle or axis");
+    return sweep_once(g, axis, false, dt, 1.0, 0);
+}
+
+int cfdart_step(cfdart_grid* g, int splitting, int n, double dt)
+{
+    if (!g || splitting < 0 || splitting > 2) return fail(CFDART_E_ARG, "bad handle or splitting");
+    int axes[3];
+    double f[3];
+    int ns = step_plan(splitting, n, axes, f);
+    for (int s = 0; s < ns; ++s) {
+        // the reference passes 0.5 * dt to the Strang x sweeps (splitting_scheme.hpp:90,116)
+        double dts = f[s] == 1.0 ? dt : f[s] * dt;
+        int rc = sweep_once(g, axes[s], false, dts, 1.0, 0);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int cfdart_compute_dt(cfdart_grid* g, int n, double t, double T, double cfl, double* dt_out, double* mass_out)
+{
+    if (!g || !dt_out) return fail(CFDART_E_ARG, "null argument");
+    cfd::GridView v = view(g);
+    hipError_t e = cfd::launch_reduce_dt(v, g->d.model, g->partials, g->st);
+    if (e != hipSuccess) return fail(CFDART_E_HIP, hipGetErrorString(e));
+    cfd::Ctrl c;
+    std::vector<double> part(2 * (size_t)g->nred);
+    HIPCHK(hipMemcpyAsync(&c, g->ctrl, sizeof(c), hipMemcpyDeviceToHost, g->st));
+    HIPCHK(hipMemcpyAsync(part.data(), g->partials, sizeof(double) * part.size(), hipMemcpyDeviceToHost, g->st));
+    HIPCHK(hipStreamSynchronize(g->st));
+    HIPCHK(hipMemsetAsync(&g->ctrl->maxu, 0, 2 * sizeof(unsigned long long), g->st));
+    double maxu, maxv;
+    std::memcpy(&maxu, &c.maxu, sizeof(double));
+    std::memcpy(&maxv, &c.maxv, sizeof(double));
+    // E/sourcecode/euler_timestep.cpp:35-43
+    double CFL = cfl;
+    if (n < 5) CFL = cfd::smin(CFL, 0.2);
+    double dt = CFL * cfd::smin(g->d.dx / maxu, g->d.dy / maxv);
+    if (t + dt > T) dt = T - t;
+    *dt_out = dt;
+    if (mass_out) {
+        double m1 = 0.0, m2 = 0.0;
+        for (int b = 0; b < g->nred; ++b) { m1 += part[2 * b]; m2 += part[2 * b + 1]; }
+        mass_out[0] = m1 * g->d.dx * g->d.dy;
+        if (g->d.model == CFDART_ALLAIRE) mass_out[1] = m2 * g->d.dx * g->d.dy;
+    }
+    return 0;
+}
+
+int cfdart_set_time(cfdart_grid* g, double t, int n)
+{
+    if (!g) return fail(CFDART_E_ARG, "null handle");
+    hipError_t e = cfd::launch_set_time(g->ctrl, t, n, g->st);
+    if (e != hipSuccess) return fail(CFDART_E_HIP, hipGetErrorString(e));
+    return 0;
+}
+
+int cfdart_run(cfdart_grid* g, int splitting, int max_steps, double T, double cfl, double* t_out, int* n_out)
+{
+    if (!g || splitting < 0 || splitting > 2 || max_steps < 0) return fail(CFDART_E_ARG, "bad argument");
+    if (!(cfl > 0.0)) return fail(CFDART_E_ARG, "cfl must be positive");
+    cfd::GridView v = view(g);
+    hipError_t e = cfd::launch_set_run(g->ctrl, T, cfl, g->st);
+    if (e == hipSuccess) e = cfd::launch_reduce_dt(v, g->d.model, g->partials, g->st);
+    if (e == hipSuccess) e = cfd::launch_init_dt(g->ctrl, g->d.dx, g->d.dy, g->st);
+    if (e != hipSuccess) return fail(CFDART_E_HIP, hipGetErrorString(e));
+    // The alternating splitting needs n on the host to pick the sweep order; fixed-order
+    // schemes do not. For alternating, step parity is tracked from the handle's n.
+    int n0 = 0;
+    if (splitting == CFDART_ALTERNATING) {
+        cfd::Ctrl c;
+        HIPCHK(hipMemcpyAsync(&c, g->ctrl, sizeof(c), hipMemcpyDeviceToHost, g->st));
+        HIPCHK(hipStreamSynchronize(g->st));
+        n0 = c.n;
+    }
+    for (int k = 0; k < max_steps; ++k) {
+        int axes[3];
+        double f[3];
+        int ns = step_plan(splitting, n0 + k, axes, f);
+        for (int s = 0; s < ns; ++s) {
+            int rc = sweep_once(g, axes[s], s == ns - 1, 0.0, f[s], 1);
+            if (rc) return rc;
+        }
+        e = cfd::launch_finalize(g->ctrl, g->d.dx, g->d.dy, g->st);
+        if (e != hipSuccess) return fail(CFDART_E_HIP, hipGetErrorString(e));
+    }
+    if (t_out || n_out) {
+        cfd::Ctrl c;
+        HIPCHK(hipMemcpyAsync(&c, g->ctrl, sizeof(c), hipMemcpyDeviceToHost, g->st));
+        HIPCHK(hipStreamSynchronize(g->st));
+        if (t_out) *t_out = c.t;
+        if (n_out) *n_out = c.n;
+        if (c.error) return fail(CFDART_E_UNPHYS, "an Allaire state failed the physical-state test");
+    }
+    return 0;
+}
+
+int cfdart_stats(cfdart_grid* g, long long* counters)
+{
+    if (!g || !counters) return fail(CFDART_E_ARG, "null argument");
+    cfd::Ctrl c;
+    HIPCHK(hipMemcpyAsync(&c, g->ctrl, sizeof(c), hipMemcpyDeviceToHost, g->st));
+    HIPCHK(hipStreamSynchronize(g->st));
+    for (int i = 0; i < 4; ++i) counters[i] = (long long)c.cnt[i];
+    return 0;
+}
+
+int cfdart_sync(cfdart_grid* g)
+{
+    if (!g) return fail(CFDART_E_ARG, "null handle");
+    HIPCHK(hipStreamSynchronize(g->st));
+    return 0;
+}
+
+void* cfdart_stream(cfdart_grid* g) { return g ? (void*)g->st : nullptr; }
+
+int cfdart_set_profiling(cfdart_grid* g, int enable)
+{
+    if (!g) return fail(CFDART_E_ARG, "null handle");
+    HIPCHK(hipStreamSynchronize(g->st));
+    harvest_profile(g);
+    for (int i = 0; i < 3; ++i) { g->prof_ms[i] = 0.0; g->prof_n[i] = 0; }
+    g->profiling = enable ? 1 : 0;
+    return 0;
+}
+
+int cfdart_profile(cfdart_grid* g, double* ms_total, long long* launches)
+{
+    if (!g || !ms_total || !launches) return fail(CFDART_E_ARG, "null argument");
+    HIPCHK(hipStreamSynchronize(g->st));
+    harvest_profile(g);
+    for (int i = 0; i < 2; ++i) { ms_total[i] = g->prof_ms[i]; launches[i] = g->prof_n[i]; }
+    return 0;
+}
+
+int cfdart_set_strip_rows(cfdart_grid* g, int rows)
+{
+    if (!g || rows < 1 || rows > 4096) return fail(CFDART_E_ARG, "rows must be in [1, 4096]");
+    g->yH = rows;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,500 @@
+// Per-cell and per-face numerics of the MUSCL-Hancock sweep, shared by the x- and y-sweep
+// kernels (and compilable for the host, which the CPU unit tests of the line scanner use).
+//
+// Every expression keeps the reference's left-to-right C++ evaluation order so that, built
+// with -ffp-contract=off, the results are bitwise those of the reference arithmetic:
+//   EOS            E/headers/stiffened_gas_eos.hpp:57-70, A/headers/mixturemodel.hpp:18-52
+//   state algebra  E/headers/misc_euler.hpp:20-131, A/headers/misc.hpp:31-266
+//   limiter        E/headers/flux_solver_MUSCLhancock.hpp:67-89 (+ van Leer, not in the reference)
+//   face           E/headers/flux_solver_MUSCLhancock.hpp:128-227, A/headers/flux_solver_MUSCLhancock.hpp:272-376
+//   HLLC           E/headers/riemann_solver_HLLC.hpp:50-145, A/headers/riemann_solver_HLLC.hpp:49-150
+//   z update       A/headers/zupdate.hpp:58-73
+// std::min(a,b) is (b<a)?b:a and std::max(a,b) is (a<b)?b:a; fmin/fmax are never used because
+// they differ on signed zeros and NaN.
+//
+// Restructuring relative to the reference (values unchanged): the reference evaluates each
+// face from a 4-cell stencil, so each cell's primitives, slope, half-step states and their
+// positivity tests are computed four or two times. Here they are computed once per cell
+// ("cell stage") and each face combines the right-hand state of its left cell with the
+// left-hand state of its right cell ("face stage"). The per-face slope fallback of the
+// reference (both slopes of a face are zeroed when either extrapolation is unphysical) is
+// reproduced by re-evaluating the two unsloped states for that face only.
+#pragma once
+
+#include <math.h>
+
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+#define CFD_HD __host__ __device__ __forceinline__
+#else
+#define CFD_HD inline
+#endif
+
+namespace cfd {
+
+enum { EULER = 0, ALLAIRE = 1 };
+
+template <int M> struct Model;
+template <> struct Model<EULER> {
+    static constexpr int NC = 4;
+    static constexpr int MN = 1;  // normal momentum component (x frame)
+    static constexpr int MT = 2;  // tangential momentum component
+};
+template <> struct Model<ALLAIRE> {
+    static constexpr int NC = 6;
+    static constexpr int MN = 2;
+    static constexpr int MT = 3;
+};
+
+// Material constants. g1m1 = gamma1 - 1.0 and g1pinf1 = gamma1 * pinf1 are exactly the
+// sub-expressions the reference recomputes on every call.
+struct Material {
+    double g1, pinf1, g2, pinf2;
+    double g1m1, g2m1, g1pinf1;
+};
+
+CFD_HD Material make_material(double g1, double pinf1, double g2, double pinf2)
+{
+    Material m;
+    m.g1 = g1; m.pinf1 = pinf1; m.g2 = g2; m.pinf2 = pinf2;
+    m.g1m1 = g1 - 1.0; m.g2m1 = g2 - 1.0; m.g1pinf1 = g1 * pinf1;
+    return m;
+}
+
+struct SchemeOpts {
+    int limiter;      // 0 minmod (reference), 1 van Leer
+    int wave_speeds;  // 0 Garrick-Owkes-Regele (reference), 1 PVRS (Allaire)
+    double z_lo, z_hi;
+};
+
+CFD_HD double smin(double a, double b) { return (b < a) ? b : a; }
+CFD_HD double smax(double a, double b) { return (a < b) ? b : a; }
+
+// ---------------------------------------------------------------- EOS
+
+// Mixture eps and k (A/headers/mixturemodel.hpp:20-22), recomputed per call there; here per z.
+struct Mix { double eps, k; };
+CFD_HD Mix mixture(const Material& m, double z)
+{
+    Mix r;
+    r.eps = z / m.g1m1 + (1.0 - z) / m.g2m1;
+    r.k = (z * m.g1 * m.pinf1 / m.g1m1 + (1.0 - z) * m.g2 * m.pinf2 / m.g2m1) / r.eps;
+    return r;
+}
+
+template <int M> struct Eos;
+template <> struct Eos<EULER> {
+    // E/headers/stiffened_gas_eos.hpp:57-70
+    CFD_HD static double pressure(const Material& m, double rho, double e) { return m.g1m1 * rho * e - m.g1pinf1; }
+    CFD_HD static double specific_ie(const Material& m, double rho, double p) { return (p + m.g1pinf1) / (m.g1m1 * rho); }
+    CFD_HD static double soundspeed(const Material& m, double rho, double p) { return sqrt((m.g1 * (p + m.pinf1)) / rho); }
+};
+struct MixEos {
+    // A/headers/mixturemodel.hpp:18-52
+    CFD_HD static double pressure(const Mix& x, double rho, double e) { return rho * e / x.eps - x.k; }
+    CFD_HD static double specific_ie(const Mix& x, double rho, double p) { return (p + x.k) * x.eps / rho; }
+    CFD_HD static double soundspeed(const Mix& x, double rho, double p) { return sqrt((x.k + p * ((1.0 / x.eps) + 1.0)) / rho); }
+    CFD_HD static double shock_jump(const Mix& x, double p, double rho, double pstar)
+    {
+        double p_avg = 0.5 * (pstar + p);
+        return sqrt((pstar + x.k + p_avg / x.eps) / rho);
+    }
+};
+
+// ---------------------------------------------------------------- state algebra
+
+// Conserved -> primitive (E/headers/misc_euler.hpp:63-78, A/headers/misc.hpp:130-147).
+// Also returns the sound speed, which the Jacobian of the same state needs
+// (E/headers/misc_euler.hpp:110, A/headers/misc.hpp:198 evaluate it from the same inputs).
+template <int M> struct Prim;
+template <> struct Prim<EULER> {
+    double W[4];
+    double c;
+    CFD_HD void from_cons(const Material& m, const double* U)
+    {
+        double rho = U[0];
+        double u = U[1] / rho;
+        double v = U[2] / rho;
+        double e = U[3] / rho - 0.5 * (u * u + v * v);
+        double p = Eos<EULER>::pressure(m, rho, e);
+        W[0] = rho; W[1] = u; W[2] = v; W[3] = p;
+        c = Eos<EULER>::soundspeed(m, rho, p);
+    }
+};
+template <> struct Prim<ALLAIRE> {
+    double W[6];
+    double c;
+    CFD_HD void from_cons(const Material& m, const double* U)
+    {
+        double rho = U[0] + U[1];
+        double u = U[2] / rho;
+        double v = U[3] / rho;
+        double e = U[4] / rho - 0.5 * (u * u + v * v);
+        Mix x = mixture(m, U[5]);
+        double p = MixEos::pressure(x, rho, e);
+        W[0] = U[0]; W[1] = U[1]; W[2] = u; W[3] = v; W[4] = p; W[5] = U[5];
+        c = MixEos::soundspeed(x, rho, p);
+    }
+};
+
+// Primitive -> conserved (E/headers/misc_euler.hpp:82-97, A/headers/misc.hpp:149-163).
+CFD_HD void p2c(const Material& m, const double* W, double* U, Model<EULER>*)
+{
+    double rho = W[0], u = W[1], v = W[2], p = W[3];
+    double e = Eos<EULER>::specific_ie(m, rho, p);
+    U[0] = rho;
+    U[1] = rho * u;
+    U[2] = rho * v;
+    U[3] = rho * e + 0.5 * rho * (u * u + v * v);
+}
+CFD_HD void p2c(const Material& m, const double* W, double* U, Model<ALLAIRE>*)
+{
+    double rho = W[0] + W[1];
+    Mix x = mixture(m, W[5]);
+    double e = MixEos::specific_ie(x, rho, W[4]);
+    U[0] = W[0];
+    U[1] = W[1];
+    U[2] = rho * W[2];
+    U[3] = rho * W[3];
+    U[4] = rho * e + 0.5 * rho * (W[2] * W[2] + W[3] * W[3]);
+    U[5] = W[5];
+}
+template <int M> CFD_HD void p2c(const Material& m, const double* W, double* U)
+{
+    p2c(m, W, U, (Model<M>*)nullptr);
+}
+
+// Allaire snapping of the non-const is_physical_state overload (A/headers/misc.hpp:253-256).
+CFD_HD void snap(double* U)
+{
+    if (fabs(U[0]) < 1e-12) U[0] = 0.0;
+    if (fabs(U[1]) < 1e-12) U[1] = 0.0;
+    if (fabs(U[5]) < 1e-12) U[5] = 0.0;
+    if (fabs(U[5] - 1.0) < 1e-12) U[5] = 1.0;
+}
+
+// Physical-state test (E/headers/misc_euler.hpp:123-131, A/headers/misc.hpp:228-237). Euler
+// has no snapping; for Allaire the caller snaps first where the reference does.
+CFD_HD bool is_physical(const double* U, Model<EULER>*)
+{
+    double rho = U[0];
+    double u = U[1] / rho;
+    double v = U[2] / rho;
+    double e = U[3] / rho - 0.5 * (u * u + v * v);
+    return rho >= 0.0 && e >= 0.0;
+}
+CFD_HD bool is_physical(const double* U, Model<ALLAIRE>*)
+{
+    double rho = U[0] + U[1];
+    double u = U[2] / rho;
+    double v = U[3] / rho;
+    double e = U[4] / rho - 0.5 * (u * u + v * v);
+    double z = U[5];
+    return U[0] >= 0.0 && U[1] >= 0.0 && e >= 0.0 && z >= 0.0 && z <= 1.0;
+}
+template <int M> CFD_HD bool is_physical(const double* U) { return is_physical(U, (Model<M>*)nullptr); }
+
+// ---------------------------------------------------------------- limiter
+
+CFD_HD double limit(int limiter, double a, double b)
+{
+    if (limiter == 1) return (a * b > 0.0) ? (2.0 * a * b) / (a + b) : 0.0;
+    if (b > 0.0) return smax(0.0, smin(a, b));
+    return smin(0.0, smax(a, b));
+}
+
+// ---------------------------------------------------------------- one side of a face
+
+// A Riemann-problem input state with the quantities HLLC derives from it
+// (E/headers/riemann_solver_HLLC.hpp:55-67): rho, normal velocity u, p, c; z for Allaire.
+// 'ok' is the physical-state test of the evolved state (flux solver step M5).
+template <int M> struct Side {
+    static constexpr int NC = Model<M>::NC;
+    double U[NC];
+    double rho, u, p, c;
+    bool ok;
+};
+
+// From a conserved state: HLLC's own primitive recovery (identical to the physical-state
+// test's divisions, which the compiler shares).
+CFD_HD void side_from_cons(const Material& m, Side<EULER>& s)
+{
+    const double* U = s.U;
+    double rho = U[0];
+    double u = U[1] / rho;
+    double v = U[2] / rho;
+    double e = U[3] / rho - 0.5 * (u * u + v * v);
+    double p = Eos<EULER>::pressure(m, rho, e);
+    s.rho = rho; s.u = u; s.p = p;
+    s.c = Eos<EULER>::soundspeed(m, rho, p);
+    s.ok = rho >= 0.0 && e >= 0.0;
+}
+CFD_HD void side_from_cons(const Material& m, Side<ALLAIRE>& s)
+{
+    const double* U = s.U;
+    double rho = U[0] + U[1];
+    double u = U[2] / rho;
+    double v = U[3] / rho;
+    double e = U[4] / rho - 0.5 * (u * u + v * v);
+    double z = U[5];
+    Mix x = mixture(m, z);
+    double p = MixEos::pressure(x, rho, e);
+    s.rho = rho; s.u = u; s.p = p;
+    s.c = MixEos::soundspeed(x, rho, p);
+    s.ok = U[0] >= 0.0 && U[1] >= 0.0 && e >= 0.0 && z >= 0.0 && z <= 1.0;
+}
+
+// Evolved state from its primitive vector T (step M4 result): p2c, then the M5 test; the
+// Allaire test snaps the state in place and the snapped state is what HLLC receives.
+template <int M> CFD_HD void side_from_prim(const Material& m, const double* T, Side<M>& s)
+{
+    p2c<M>(m, T, s.U);
+    if (M == ALLAIRE) snap(s.U);
+    side_from_cons(m, s);
+}
+
+// The first-order fallback state (M5 failed): the raw stencil cell, unsnapped.
+template <int M> CFD_HD void side_from_raw(const Material& m, const double* U, Side<M>& s)
+{
+    for (int k = 0; k < Model<M>::NC; ++k) s.U[k] = U[k];
+    side_from_cons(m, s);
+}
+
+// ---------------------------------------------------------------- cell stage
+
+// Everything about one cell that the reference recomputes per face: slope-extrapolation
+// positivity (M3) for both faces, and the two sloped half-step states (M4, M5).
+template <int M> struct CellStage {
+    static constexpr int NC = Model<M>::NC;
+    bool fR, fL;       // M3: p2c(W + del/2), p2c(W - del/2) physical
+    Side<M> R, L;      // sloped evolved states at the cell's right and left faces
+};
+
+// tmp = (s * A(W)) * del, summed over k = 0.. in order (Eigen's unrolled lazy product of a
+// fixed-size matrix and a vector: E/headers/flux_solver_MUSCLhancock.hpp:197,201).
+CFD_HD void jac_times(const double* W, double c, double s, const double* d, double* tmp, Model<EULER>*)
+{
+    double rho = W[0], u = W[1];
+    double a00 = s * u, a01 = s * rho, z0 = s * 0.0;
+    double a13 = s * (1.0 / rho);
+    double a31 = s * (rho * c * c);
+    tmp[0] = z0 * d[3] + (z0 * d[2] + (a01 * d[1] + a00 * d[0]));
+    tmp[1] = a13 * d[3] + (z0 * d[2] + (a00 * d[1] + z0 * d[0]));
+    tmp[2] = z0 * d[3] + (a00 * d[2] + (z0 * d[1] + z0 * d[0]));
+    tmp[3] = a00 * d[3] + (z0 * d[2] + (a31 * d[1] + z0 * d[0]));
+}
+CFD_HD void jac_times(const double* W, double c, double s, const double* d, double* tmp, Model<ALLAIRE>*)
+{
+    double rho = W[0] + W[1];
+    double su = s * W[2], z0 = s * 0.0;
+    double a02 = s * W[0], a12 = s * W[1];
+    double a24 = s * (1.0 / rho);
+    double a42 = s * (rho * c * c);
+    tmp[0] = z0 * d[5] + (z0 * d[4] + (z0 * d[3] + (a02 * d[2] + (z0 * d[1] + su * d[0]))));
+    tmp[1] = z0 * d[5] + (z0 * d[4] + (z0 * d[3] + (a12 * d[2] + (su * d[1] + z0 * d[0]))));
+    tmp[2] = z0 * d[5] + (a24 * d[4] + (z0 * d[3] + (su * d[2] + (z0 * d[1] + z0 * d[0]))));
+    tmp[3] = z0 * d[5] + (z0 * d[4] + (su * d[3] + (z0 * d[2] + (z0 * d[1] + z0 * d[0]))));
+    tmp[4] = z0 * d[5] + (su * d[4] + (z0 * d[3] + (a42 * d[2] + (z0 * d[1] + z0 * d[0]))));
+    tmp[5] = su * d[5] + (z0 * d[4] + (z0 * d[3] + (z0 * d[2] + (z0 * d[1] + z0 * d[0]))));
+}
+
+// M3 test of one extrapolated state (Allaire: the snapping overload on a temporary).
+template <int M> CFD_HD bool extrap_physical(const Material& m, const double* T)
+{
+    double U[Model<M>::NC];
+    p2c<M>(m, T, U);
+    if (M == ALLAIRE) snap(U);
+    return is_physical<M>(U);
+}
+
+// Cell stage from the cell's primitives W (with c) and its two neighbours' primitives.
+template <int M>
+CFD_HD void cell_stage(const Material& m, int limiter, double s, const Prim<M>& P,
+                       const double* Wm, const double* Wp, CellStage<M>& cs)
+{
+    constexpr int NC = Model<M>::NC;
+    const double* W = P.W;
+    double del[NC], TR[NC], TL[NC], tmp[NC];
+    for (int k = 0; k < NC; ++k) del[k] = limit(limiter, W[k] - Wm[k], Wp[k] - W[k]);
+    for (int k = 0; k < NC; ++k) { TR[k] = W[k] + 0.5 * del[k]; TL[k] = W[k] - 0.5 * del[k]; }
+    cs.fR = extrap_physical<M>(m, TR);
+    cs.fL = extrap_physical<M>(m, TL);
+    jac_times(W, P.c, s, del, tmp, (Model<M>*)nullptr);
+    for (int k = 0; k < NC; ++k) { TR[k] = TR[k] - tmp[k]; TL[k] = TL[k] - tmp[k]; }
+    side_from_prim<M>(m, TR, cs.R);
+    side_from_prim<M>(m, TL, cs.L);
+}
+
+// Unsloped half-step state (slopes zeroed by M3): W + 0.5*0 - (s*A)*0, evaluated literally.
+template <int M>
+CFD_HD void unsloped_side(const Material& m, double s, const Prim<M>& P, bool right, Side<M>& out)
+{
+    constexpr int NC = Model<M>::NC;
+    double del[NC], T[NC], tmp[NC];
+    for (int k = 0; k < NC; ++k) del[k] = 0.0;
+    jac_times(P.W, P.c, s, del, tmp, (Model<M>*)nullptr);
+    for (int k = 0; k < NC; ++k) T[k] = (right ? P.W[k] + 0.5 * del[k] : P.W[k] - 0.5 * del[k]) - tmp[k];
+    side_from_prim<M>(m, T, out);
+}
+
+// ---------------------------------------------------------------- HLLC
+
+template <int M> struct FaceOut {
+    double F[Model<M>::NC];
+    double ustar, zstar;
+};
+
+// HLLC flux from two prepared sides (E/headers/riemann_solver_HLLC.hpp:69-144,
+// A/headers/riemann_solver_HLLC.hpp:70-149) and z* (A/headers/flux_solver_MUSCLhancock.hpp:375).
+template <int M>
+CFD_HD void hllc(const Material& m, int wave_speeds, const Side<M>& L, const Side<M>& R, FaceOut<M>& o)
+{
+    constexpr int NC = Model<M>::NC;
+    constexpr int MN = Model<M>::MN;
+    double rhoL = L.rho, uL = L.u, pL = L.p, cL = L.c;
+    double rhoR = R.rho, uR = R.u, pR = R.p, cR = R.c;
+    double SL, SR;
+    if (M == ALLAIRE && wave_speeds == 1) {
+        double rho_avg = 0.5 * (rhoL + rhoR);
+        double c_avg = 0.5 * (cL + cR);
+        double p_pvrs = smax(0.0, 0.5 * (pL + pR) + 0.5 * (uL - uR) * rho_avg * c_avg);
+        if (p_pvrs > pL) SL = uL - MixEos::shock_jump(mixture(m, L.U[NC - 1]), pL, rhoL, p_pvrs);
+        else SL = uL - cL;
+        if (p_pvrs > pR) SR = uR + MixEos::shock_jump(mixture(m, R.U[NC - 1]), pR, rhoR, p_pvrs);
+        else SR = uR + cR;
+    } else {
+        double u_avg = 0.5 * (uL + uR);
+        double c_avg = 0.5 * (cL + cR);
+        SL = smin(u_avg - c_avg, uL - cL);
+        SR = smax(u_avg + c_avg, uR + cR);
+    }
+    double us = (pR - pL + rhoL * uL * (SL - uL) - rhoR * uR * (SR - uR))
+                / (rhoL * (SL - uL) - rhoR * (SR - uR));
+    double fL = (SL - uL) / (SL - us);
+    double fR = (SR - uR) / (SR - us);
+    double sg = copysign(1.0, us);
+    double wL = (1.0 + sg) / 2.0;
+    double wR = (1.0 - sg) / 2.0;
+    double mSL = smin(0.0, SL);
+    double MSR = smax(0.0, SR);
+    double eL = L.U[NC == 4 ? 3 : 4] + (us - uL) * (rhoL * us + (pL / (SL - uL)));
+    double eR = R.U[NC == 4 ? 3 : 4] + (us - uR) * (rhoR * us + (pR / (SR - uR)));
+    for (int k = 0; k < NC; ++k) {
+        double usl, usr, fl, fr;
+        if (M == ALLAIRE && k == 5) {
+            usl = 0.0; usr = 0.0; fl = 0.0; fr = 0.0;
+        } else if (k == MN) {
+            usl = fL * rhoL * us;
+            usr = fR * rhoR * us;
+            fl = L.U[k] * uL + pL;
+            fr = R.U[k] * uR + pR;
+        } else if (k == NC - 1 - (M == ALLAIRE ? 1 : 0)) {  // energy
+            usl = fL * eL;
+            usr = fR * eR;
+            fl = (L.U[k] + pL) * uL;
+            fr = (R.U[k] + pR) * uR;
+        } else {
+            usl = fL * L.U[k];
+            usr = fR * R.U[k];
+            fl = L.U[k] * uL;
+            fr = R.U[k] * uR;
+        }
+        o.F[k] = wL * (fl + mSL * (usl - L.U[k])) + wR * (fr + MSR * (usr - R.U[k]));
+    }
+    if (M == ALLAIRE) o.F[5] = 0.0;
+    o.ustar = us;
+    if (M == ALLAIRE) o.zstar = 0.5 * (1.0 + sg) * L.U[5] + 0.5 * (1.0 - sg) * R.U[5];
+    else o.zstar = 0.0;
+}
+
+// ---------------------------------------------------------------- face stage
+
+// Face between cell a (left) and cell b (right): M3 combination, M5 fallback, HLLC.
+// flags: bit0 slope fallback taken, bit1 evolve fallback taken.
+template <int M>
+CFD_HD int face_stage(const Material& m, const SchemeOpts& o, double s,
+                      const Prim<M>& Pa, const double* Ua, const CellStage<M>& a,
+                      const Prim<M>& Pb, const double* Ub, const CellStage<M>& b, FaceOut<M>& out)
+{
+    int flags = 0;
+    if (a.fR && b.fL) {
+        if (a.R.ok && b.L.ok) {
+            hllc<M>(m, o.wave_speeds, a.R, b.L, out);
+            return 0;
+        }
+    } else {
+        flags |= 1;
+        Side<M> L, R;
+        unsloped_side<M>(m, s, Pa, true, L);
+        unsloped_side<M>(m, s, Pb, false, R);
+        if (L.ok && R.ok) {
+            hllc<M>(m, o.wave_speeds, L, R, out);
+            return flags;
+        }
+    }
+    flags |= 2;
+    Side<M> L, R;
+    side_from_raw<M>(m, Ua, L);
+    side_from_raw<M>(m, Ub, R);
+    hllc<M>(m, o.wave_speeds, L, R, out);
+    return flags;
+}
+
+// ---------------------------------------------------------------- update
+
+// Euler repair (E/sourcecode/euler_rowsweep.cpp:47-82): 'old' and 'fut' in the unrotated frame.
+CFD_HD bool euler_repair(const Material& m, const double* old, double* fut)
+{
+    double rho = fut[0];
+    double u = fut[1] / rho;
+    double v = fut[2] / rho;
+    double e = fut[3] / rho - 0.5 * (u * u + v * v);
+    if (rho >= 0.0 && e >= 0.0) return false;
+    double rho_old = old[0];
+    double u_old = old[1] / rho_old;
+    double v_old = old[2] / rho_old;
+    double e_old = old[3] / rho_old - 0.5 * (u_old * u_old + v_old * v_old);
+    if (rho <= 1e-12) rho = rho_old;
+    if (e <= 1e-12) e = e_old;
+    double p = Eos<EULER>::pressure(m, rho, e);
+    double W[4] = {rho, u, v, p};
+    p2c<EULER>(m, W, fut);
+    return true;
+}
+
+// A/headers/zupdate.hpp:62-67 with the clamp bounds parameterised.
+CFD_HD double zupdate(double r, double z, double usL, double usR, double zsL, double zsR,
+                      double z_lo, double z_hi)
+{
+    double newz = z - r * (usR * zsR - usL * zsL - z * (usR - usL));
+    return smax(smin(z_hi, newz), z_lo);
+}
+
+// Wave speed |u|+c, |v|+c of a final (unrotated) state: compute_dt's per-cell work
+// (E/sourcecode/euler_timestep.cpp:16-24, A/sourcecode/allaire_diffuse_timestep.cpp:17-26).
+CFD_HD void wave_speeds(const Material& m, const double* U, double& su, double& sv, Model<EULER>*)
+{
+    double rho = U[0];
+    double u = U[1] / rho;
+    double v = U[2] / rho;
+    double e = U[3] / rho - 0.5 * (u * u + v * v);
+    double p = Eos<EULER>::pressure(m, rho, e);
+    double c = Eos<EULER>::soundspeed(m, rho, p);
+    su = fabs(u) + c;
+    sv = fabs(v) + c;
+}
+CFD_HD void wave_speeds(const Material& m, const double* U, double& su, double& sv, Model<ALLAIRE>*)
+{
+    double rho = U[0] + U[1];
+    double u = U[2] / rho;
+    double v = U[3] / rho;
+    double e = U[4] / rho - 0.5 * (u * u + v * v);
+    double z = U[5];
+    Mix x = mixture(m, z);
+    double p = MixEos::pressure(x, rho, e);
+    double c = MixEos::soundspeed(x, rho, p);
+    su = fabs(u) + c;
+    sv = fabs(v) + c;
+}
+
+}  // namespace cfd

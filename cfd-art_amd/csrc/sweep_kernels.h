@@ -1,0 +1,42 @@
+// Device data structures shared by the sweep kernels and the C-ABI layer.
+#pragma once
+
+#include "physics.h"
+
+namespace cfd {
+
+// Per-grid device control block: the state of the reference's time loop
+// (E/headers/simulation.hpp:48-61) kept on the device so a run of many steps needs no
+// host round trip per step.
+struct Ctrl {
+    double t, dt, T, cfl;
+    int n, active;
+    unsigned long long maxu, maxv;  // bit patterns of non-negative doubles (max-reduced)
+    unsigned long long cnt[4];      // slope fallbacks, evolve fallbacks, repairs, unphysical
+    double mass_partial_unused;
+    int error;
+    int pad;
+};
+
+// Device view of one grid: SoA planes, interior cells only (the ghost frame is produced on
+// the fly by mirrored indexing, which is the reference's pre_sweep fill).
+struct GridView {
+    const double* cur;
+    double* nxt;
+    long long plane;  // doubles between component planes
+    int pitch;        // doubles between rows
+    int nx, ny;
+    int bc[4];        // left, right, bottom, top: 0 transmissive, 1 reflective
+    Material mat;
+    SchemeOpts opt;
+    double dx, dy;
+    double dx_half_col;  // spacing of the column sweep's half-step (reference: dx)
+    Ctrl* ctrl;
+};
+
+// Sweep-kernel launch geometry.
+constexpr int XTILE = 60;         // output columns per wavefront in the x sweep (64 lanes - 4 halo)
+constexpr int XROWS_PER_WAVE = 4; // rows each x-sweep wavefront processes
+constexpr int YBLOCK = 256;       // threads (columns) per y-sweep block
+
+}  // namespace cfd

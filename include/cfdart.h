@@ -1,0 +1,118 @@
+/*
+ * cfdart: C-ABI of the MI355X (gfx950) MUSCL-Hancock sweep library.
+ *
+ * The reference (cfd-art) drives its hot path through the problem_base / splitting_scheme
+ * C++ interfaces, one grid row or column per call:
+ *   problem_base::{pre_sweep, update_row, update_col, post_sweep, compute_dt}
+ *     euler_equations/euler_simulation_code/headers/problem_base.hpp:22-39
+ *   splitting_scheme_base::advance_timestep
+ *     euler_equations/euler_simulation_code/headers/splitting_scheme.hpp:26
+ * A row is far too small a unit for a GPU, so this boundary sits one level up: whole sweeps,
+ * whole steps and whole runs over an opaque device-resident grid. Plain C types only.
+ *
+ * Conventions: every call returns 0 or a negative CFDART_E_* code and never aborts the
+ * process; cfdart_last_error() describes the last failure of the calling thread. A handle owns
+ * its device memory; the caller owns host buffers. One host thread per handle at a time; each
+ * handle has its own HIP stream; calls are asynchronous on it except those that return host
+ * data (download, compute_dt, stats, run when t_out is non-null, sync).
+ *
+ * Host state format (upload/download): the reference's grid[i][j](k) including its ghost frame
+ * (euler_equations/euler_simulation_code/headers/typedefs.hpp:19-21), row-major AoS:
+ *   host[((i * (nx + 2*ng)) + j) * ncomp + k],  i in [0, ny+2ng), j in [0, nx+2ng).
+ * Euler: k = rho, rho*u, rho*v, E (ncomp 4). Allaire: z*rho1, (1-z)*rho2, rho*u, rho*v, E, z (6).
+ */
+#ifndef CFDART_H
+#define CFDART_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CFDART_E_ARG     (-1) /* invalid argument or descriptor */
+#define CFDART_E_NOMEM   (-2) /* device or host allocation failed */
+#define CFDART_E_HIP     (-3) /* HIP runtime error */
+#define CFDART_E_UNPHYS  (-4) /* an Allaire state failed the physical-state test (reference: assert abort) */
+#define CFDART_E_NODEV   (-5) /* no usable gfx950 device */
+#define CFDART_E_STATE   (-6) /* call not valid in the handle's current state */
+
+enum { CFDART_EULER = 0, CFDART_ALLAIRE = 1 };
+enum { CFDART_TRANSMISSIVE = 0, CFDART_REFLECTIVE = 1 };
+enum { CFDART_MINMOD = 0, CFDART_VANLEER = 1 };
+enum { CFDART_GARRICK = 0, CFDART_PVRS = 1 };
+enum { CFDART_NAIVE = 0, CFDART_STRANG = 1, CFDART_ALTERNATING = 2 };
+
+/* Grid and scheme description; mirrors the reference's sim_info + SGparams/binarySGparams +
+ * the flux-solver choices (euler_equations/euler_simulation_code/headers/sim_info.hpp:16-63,
+ * headers/stiffened_gas_eos.hpp:20-50, sourcecode/euler_io.cpp:13-234,436-491). */
+typedef struct cfdart_desc {
+    int model;            /* CFDART_EULER or CFDART_ALLAIRE */
+    int nx, ny, ng;       /* interior cells (>= 2 each) and ghost width (must be 2: MUSCL) */
+    double x0, y0, dx, dy;
+    double gamma1, pinf1; /* Euler material / Allaire fluid 1 */
+    double gamma2, pinf2; /* Allaire fluid 2 (ignored for Euler) */
+    int bc[4];            /* left, right, bottom, top: CFDART_TRANSMISSIVE / CFDART_REFLECTIVE */
+    int limiter;          /* CFDART_MINMOD (reference) or CFDART_VANLEER */
+    int wave_speeds;      /* CFDART_GARRICK (reference) or CFDART_PVRS (Allaire only) */
+    int colsweep_uses_dx; /* 1 = reference: the column sweep's half-step uses dx */
+    double z_lo, z_hi;    /* Allaire volume-fraction clamp (reference 0 and 1) */
+} cfdart_desc;
+
+typedef struct cfdart_grid cfdart_grid;
+
+/* Counters (cfdart_stats): reference diagnostics that it prints instead of counting. */
+enum {
+    CFDART_CNT_SLOPE_FALLBACK = 0,  /* flux_solver_MUSCLhancock.hpp:184-189 */
+    CFDART_CNT_EVOLVE_FALLBACK = 1, /* flux_solver_MUSCLhancock.hpp:216-223 */
+    CFDART_CNT_REPAIR = 2,          /* euler_rowsweep.cpp:49-81 */
+    CFDART_CNT_UNPHYSICAL = 3       /* allaire_diffuse_rowsweep.cpp:49 (assert) */
+};
+
+const char* cfdart_last_error(void);
+int cfdart_device_count(void);
+
+/* Allocates two device state buffers and the control block. Replaces the grid allocation of
+ * set_ICs (euler_equations/euler_simulation_code/sourcecode/euler_io.cpp:494). */
+int cfdart_create(const cfdart_desc* desc, cfdart_grid** out);
+void cfdart_destroy(cfdart_grid* g);
+
+/* Interior of a host grid (ghost frame ignored) -> device; resets t, n and the counters. */
+int cfdart_upload(cfdart_grid* g, const double* host_state);
+/* Device -> host grid; writes the interior and fills the ghost frame as pre_sweep would
+ * (euler_equations/euler_simulation_code/sourcecode/euler_ppsweep.cpp:7-113; corners untouched). */
+int cfdart_download(cfdart_grid* g, double* host_state);
+
+/* One sweep: pre_sweep + update_row over all rows (axis 0) or update_col over all columns
+ * (axis 1) + post_sweep (splitting_scheme.hpp:39-50,53-66). */
+int cfdart_sweep(cfdart_grid* g, int axis, double dt);
+/* compute_dt on the current state (euler_timestep.cpp:7-44, race-free max): synchronous.
+ * mass_out (nullable) receives sum(rho)*dx*dy (Euler) or the two phase masses (Allaire). */
+int cfdart_compute_dt(cfdart_grid* g, int n, double t, double T, double cfl, double* dt_out,
+                      double* mass_out);
+/* One time step with a host-given dt: splitting_scheme::advance_timestep
+ * (splitting_scheme.hpp:37-67 naive, :78-122 Strang; alternating = XY on even n, YX on odd n). */
+int cfdart_step(cfdart_grid* g, int splitting, int n, double dt);
+/* The time loop of simulation::run_simulation (simulation.hpp:48-61) on the device: from the
+ * handle's (t, n) take at most max_steps steps while t < T, each with dt = compute_dt(...).
+ * The loop state stays on the device; with t_out/n_out non-null the call synchronises and
+ * reports the final t and n. Pass T = INFINITY for a fixed number of steps. */
+int cfdart_run(cfdart_grid* g, int splitting, int max_steps, double T, double cfl,
+               double* t_out, int* n_out);
+/* Sets the loop state used by cfdart_run (defaults after upload: t = 0, n = 0). */
+int cfdart_set_time(cfdart_grid* g, double t, int n);
+/* Counters accumulated since upload (see CFDART_CNT_*), 4 values; synchronous. */
+int cfdart_stats(cfdart_grid* g, long long* counters);
+int cfdart_sync(cfdart_grid* g);
+/* The HIP stream (hipStream_t) the handle's kernels run on. */
+void* cfdart_stream(cfdart_grid* g);
+/* Kernel timing: with profiling enabled every sweep launch is bracketed by HIP events on the
+ * handle's stream; cfdart_profile returns, per axis (x, y), the summed kernel time in ms and
+ * the number of launches since profiling was (re-)enabled. Synchronous. */
+int cfdart_set_profiling(cfdart_grid* g, int enable);
+int cfdart_profile(cfdart_grid* g, double* ms_total /* [2] */, long long* launches /* [2] */);
+/* Tuning: rows per y-sweep strip (default 32). */
+int cfdart_set_strip_rows(cfdart_grid* g, int rows);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
