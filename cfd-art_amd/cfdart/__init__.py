@@ -5,7 +5,7 @@ this module is the thin plumbing that tests and the benchmark use to call it. It
 back to a CPU implementation: if the library or a GPU is missing, calls raise.
 """
 from ._lib import (  # noqa: F401
-    CfdartError, Desc, DiscParams, BubbleParams, Grid, lib, library_path,
+    CfdartError, Desc, DiscParams, BubbleParams, Grid, Batch, time_tile, lib, library_path,
     EULER, ALLAIRE, TRANSMISSIVE, REFLECTIVE, MINMOD, VANLEER, GARRICK, PVRS,
     NAIVE, STRANG, ALTERNATING, ncomp, make_desc, random_discs, shocked_bubbles, planar,
     prim_to_cons, device_count,

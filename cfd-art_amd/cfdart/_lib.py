@@ -85,6 +85,29 @@ def lib():
         L.cfdart_ic_shocked_bubbles.argtypes = [P(Desc), P(BubbleParams), dp]
         L.cfdart_ic_planar.argtypes = [P(Desc), dp, dp, C.c_double, C.c_double, C.c_double, dp]
         L.cfdart_prim_to_cons.argtypes = [P(Desc), dp, dp]
+        L.cfdart_set_tile.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        # batch (include/cfdart_batch.h)
+        L.cfdart_batch_create.argtypes = [C.c_int, C.c_size_t, P(C.c_void_p)]
+        L.cfdart_batch_destroy.argtypes = [C.c_void_p]
+        L.cfdart_batch_destroy.restype = None
+        L.cfdart_batch_add.argtypes = [C.c_void_p, P(Desc), dp, C.c_int, C.c_int, C.c_double, C.c_double,
+                                       C.c_int, C.c_int, P(C.c_int)]
+        L.cfdart_batch_remove.argtypes = [C.c_void_p, C.c_int]
+        L.cfdart_batch_step.argtypes = [C.c_void_p, C.c_int]
+        L.cfdart_batch_download.argtypes = [C.c_void_p, C.c_int, dp]
+        L.cfdart_batch_status.argtypes = [C.c_void_p, C.c_int, dp, P(C.c_int), P(C.c_int)]
+        L.cfdart_batch_stats.argtypes = [C.c_void_p, C.c_int, P(C.c_longlong)]
+        L.cfdart_batch_sync.argtypes = [C.c_void_p]
+        L.cfdart_batch_info.argtypes = [C.c_void_p, P(C.c_int), P(C.c_size_t), P(C.c_int)]
+        L.cfdart_batch_frame.argtypes = [C.c_void_p, C.c_int, dp]
+        L.cfdart_batch_frames_done.argtypes = [C.c_void_p]
+        L.cfdart_batch_stream.argtypes = [C.c_void_p]
+        L.cfdart_batch_stream.restype = C.c_void_p
+        L.cfdart_host_alloc.argtypes = [C.c_size_t]
+        L.cfdart_host_alloc.restype = C.c_void_p
+        L.cfdart_host_free.argtypes = [C.c_void_p]
+        L.cfdart_host_free.restype = None
+        L.cfdart_time_tile.argtypes = [P(Desc), C.c_int, C.c_int, C.c_int, dp, dp]
         _lib = L
     return _lib
 
@@ -227,5 +250,94 @@ class Grid:
         _check(lib().cfdart_profile(self.h, ms, n))
         return {"x": (ms[0], n[0]), "y": (ms[1], n[1])}
 
+    def set_tile(self, y_strip_rows, x_rows_per_wave):
+        _check(lib().cfdart_set_tile(self.h, y_strip_rows, x_rows_per_wave))
+
     def set_strip_rows(self, rows):
         _check(lib().cfdart_set_strip_rows(self.h, rows))
+
+
+class Batch:
+    """Many grids advancing together (include/cfdart_batch.h)."""
+
+    def __init__(self, capacity, mem_budget_bytes=0):
+        h = C.c_void_p()
+        _check(lib().cfdart_batch_create(capacity, mem_budget_bytes, C.byref(h)))
+        self.h = h
+        self.descs = {}
+
+    def close(self):
+        if self.h:
+            lib().cfdart_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, desc, state, splitting=ALTERNATING, max_steps=1, T=math.inf, cfl=0.8, y_strip_rows=0,
+            x_rows_per_wave=0):
+        """Returns the slot id; raises CfdartError (code -2) when admission is refused."""
+        state = np.ascontiguousarray(state, dtype=np.float64)
+        slot = C.c_int()
+        _check(lib().cfdart_batch_add(self.h, C.byref(desc), _dptr(state), splitting, max_steps, T, cfl,
+                                      y_strip_rows, x_rows_per_wave, C.byref(slot)))
+        self.descs[slot.value] = desc
+        return slot.value
+
+    def try_add(self, *a, **k):
+        try:
+            return self.add(*a, **k)
+        except CfdartError as e:
+            if e.code == -2:
+                return None
+            raise
+
+    def remove(self, slot):
+        _check(lib().cfdart_batch_remove(self.h, slot))
+        self.descs.pop(slot, None)
+
+    def step(self, nsteps=1):
+        _check(lib().cfdart_batch_step(self.h, nsteps))
+
+    def sync(self):
+        _check(lib().cfdart_batch_sync(self.h))
+
+    def download(self, slot):
+        out = empty_state(self.descs[slot])
+        _check(lib().cfdart_batch_download(self.h, slot, _dptr(out)))
+        return out
+
+    def status(self, slot):
+        t = C.c_double()
+        n = C.c_int()
+        a = C.c_int()
+        _check(lib().cfdart_batch_status(self.h, slot, C.byref(t), C.byref(n), C.byref(a)))
+        return t.value, n.value, bool(a.value)
+
+    def stats(self, slot):
+        c = (C.c_longlong * 4)()
+        _check(lib().cfdart_batch_stats(self.h, slot, c))
+        return list(c)
+
+    def info(self):
+        live = C.c_int()
+        b = C.c_size_t()
+        t = C.c_int()
+        _check(lib().cfdart_batch_info(self.h, C.byref(live), C.byref(b), C.byref(t)))
+        return live.value, b.value, t.value
+
+    def frame(self, slot, host_ptr):
+        _check(lib().cfdart_batch_frame(self.h, slot, C.cast(host_ptr, C.POINTER(C.c_double))))
+
+    def frames_done(self):
+        _check(lib().cfdart_batch_frames_done(self.h))
+
+
+def time_tile(desc, y_strip_rows, x_rows_per_wave, reps=5):
+    ux = C.c_double()
+    uy = C.c_double()
+    _check(lib().cfdart_time_tile(C.byref(desc), y_strip_rows, x_rows_per_wave, reps, C.byref(ux), C.byref(uy)))
+    return ux.value, uy.value

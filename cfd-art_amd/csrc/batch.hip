@@ -1,0 +1,701 @@
+// Multi-grid execution (include/cfdart_batch.h): a persistent kernel per sweep over a device
+// work-list of tiles from every live grid, per-grid time-loop state on the device, and a HIP
+// graph per step.
+//
+// Buffer roles: at a step boundary each grid's state is in buf[0]; the first sweep of a step
+// reads buf[0] and writes buf[1], the second reads buf[1] and writes buf[0]. A grid whose loop
+// has ended is skipped by both sweeps, so its state stays in buf[0]. The sweep axis of each
+// phase follows the grid's splitting (alternating: x then y on even steps, y then x on odd).
+#include "cfdart_batch.h"
+#include "cfdart_host.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "sweep_device.h"
+
+namespace cfd {
+void set_last_error(const std::string& msg);  // cfdart_api.hip
+}
+
+namespace cfd {
+
+struct BatchGrid {
+    double* buf[2];
+    long long plane;
+    int pitch, nx, ny;
+    int model, splitting;
+    int bc[4];
+    Material mat;
+    SchemeOpts opt;
+    double dx, dy, dx_half_col;
+    int yH, xrows;
+    int steps_left;
+    int live;
+    Ctrl ctrl;
+};
+
+struct Tile {
+    int grid, axis, a, b;  // x: (column tile, row group); y: (column block, strip)
+};
+
+// Work-list heads: one queue per model (Euler and Allaire tiles run in separate kernels so each
+// keeps its own register budget) and per sweep phase.
+struct BatchQueue {
+    unsigned int head[2][2];  // [model][phase]
+    int ntiles[2];            // [model]
+};
+
+__device__ __forceinline__ GridView batch_view(BatchGrid* bg, int phase)
+{
+    GridView g;
+    g.cur = bg->buf[phase];
+    g.nxt = bg->buf[1 - phase];
+    g.plane = bg->plane;
+    g.pitch = bg->pitch;
+    g.nx = bg->nx;
+    g.ny = bg->ny;
+    for (int i = 0; i < 4; ++i) g.bc[i] = bg->bc[i];
+    g.mat = bg->mat;
+    g.opt = bg->opt;
+    g.dx = bg->dx;
+    g.dy = bg->dy;
+    g.dx_half_col = bg->dx_half_col;
+    g.ctrl = &bg->ctrl;
+    return g;
+}
+
+// One sweep phase of every live grid of model M. Workgroups pull tiles from the queue until
+// it is empty, so uneven grids balance across the chip.
+template <int M, bool LAST>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void batch_sweep_kernel(
+    BatchGrid* grids, const Tile* tiles, BatchQueue* q, int phase)
+{
+    __shared__ int s_t;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_t = (int)atomicAdd(&q->head[M][phase], 1u);
+        __syncthreads();
+        const int t = s_t;
+        if (t >= q->ntiles[M]) break;
+        const Tile tl = tiles[t];
+        BatchGrid* bg = grids + tl.grid;
+        if (!bg->live || !bg->ctrl.active) continue;
+        const int axis = (bg->splitting == 2 && (bg->ctrl.n & 1)) ? 1 - phase : phase;
+        if (tl.axis != axis) continue;
+        const GridView g = batch_view(bg, phase);
+        const double dt = bg->ctrl.dt;
+        Tally ta;
+        if (axis == 0) {
+            const int xrows = bg->xrows;
+            const int row0 = (tl.b * 4 + (threadIdx.x >> 6)) * xrows;
+            x_tile<M, LAST>(g, dt, tl.a, row0, xrows, ta);
+        } else {
+            const int y0 = tl.b * bg->yH, y1 = min(y0 + bg->yH, bg->ny);
+            y_tile<M, LAST>(g, dt, tl.a, y0, y1, ta);
+        }
+        ta.flush(&bg->ctrl, LAST);
+    }
+}
+
+// End of a step for every live grid (E/headers/simulation.hpp:68-70 per grid, plus the
+// step budget), and the queue reset for the next step.
+__global__ void batch_finalize_kernel(BatchGrid* grids, int cap, BatchQueue* q)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        q->head[0][0] = 0;
+        q->head[0][1] = 0;
+        q->head[1][0] = 0;
+        q->head[1][1] = 0;
+    }
+    if (i >= cap) return;
+    BatchGrid* bg = grids + i;
+    Ctrl* c = &bg->ctrl;
+    if (!bg->live || !c->active) return;
+    double t = c->t + c->dt;
+    int n = c->n + 1;
+    if (fabs(t - c->T) < 1e-12) t = c->T;
+    c->t = t;
+    c->n = n;
+    bg->steps_left -= 1;
+    c->active = (t < c->T) && bg->steps_left > 0;
+    if (c->active) c->dt = dt_from_maxima(c, n, t, bg->dx, bg->dy);
+    c->maxu = 0;
+    c->maxv = 0;
+}
+
+__global__ void batch_reset_queue_kernel(BatchQueue* q, int ntiles_euler, int ntiles_allaire)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        for (int m = 0; m < 2; ++m) q->head[m][0] = q->head[m][1] = 0;
+        q->ntiles[0] = ntiles_euler;
+        q->ntiles[1] = ntiles_allaire;
+    }
+}
+
+// After the reduction of a newly admitted grid: its first dt and loop test.
+__global__ void batch_admit_kernel(BatchGrid* bg)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    Ctrl* c = &bg->ctrl;
+    c->active = (c->t < c->T) && bg->steps_left > 0;
+    if (c->active) c->dt = dt_from_maxima(c, c->n, c->t, bg->dx, bg->dy);
+    c->maxu = 0;
+    c->maxv = 0;
+}
+
+// Frame snapshot: contiguous copy of the interior of buf[0] (SoA planes, pitch removed).
+__global__ void batch_snapshot_kernel(const BatchGrid* bg, double* dst)
+{
+    const int nc = bg->model == ALLAIRE ? 6 : 4;
+    const long long n = (long long)bg->nx * bg->ny;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n * nc;
+         i += (long long)gridDim.x * blockDim.x) {
+        long long k = i / n, c = i % n;
+        long long y = c / bg->nx, x = c % bg->nx;
+        dst[i] = bg->buf[0][k * bg->plane + y * bg->pitch + x];
+    }
+}
+
+}  // namespace cfd
+
+using cfd::BatchGrid;
+using cfd::BatchQueue;
+using cfd::Tile;
+
+struct cfdart_batch {
+    int cap;
+    size_t budget, used;
+    BatchGrid* d_grids;
+    std::vector<BatchGrid> h_grids;
+    std::vector<char> live;
+    std::vector<size_t> bytes;
+    Tile* d_tiles[2];  // per model
+    size_t tile_cap[2];
+    int ntile[2];
+    BatchQueue* d_q;
+    double* d_partials;
+    int nred;
+    hipStream_t st, st2, copy_st;
+    hipEvent_t fork_ev, join_ev;
+    hipGraphExec_t exec;
+    bool have_graph;
+    int nblocks[2];
+    int ntiles;
+    std::multimap<size_t, double*> pool;  // freed state buffers for reuse, by size
+    double* d_snap;
+    size_t snap_cap;
+    hipEvent_t snap_ev;
+};
+
+namespace {
+
+int bfail(int code, const std::string& m)
+{
+    cfd::set_last_error(m);
+    return code;
+}
+
+#define BCHK(x)                                                                                 \
+    do {                                                                                        \
+        hipError_t e_ = (x);                                                                    \
+        if (e_ != hipSuccess) return bfail(CFDART_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+size_t round_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+double* pool_get(cfdart_batch* b, size_t bytes)
+{
+    auto it = b->pool.find(bytes);
+    if (it != b->pool.end()) {
+        double* p = it->second;
+        b->pool.erase(it);
+        return p;
+    }
+    double* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        // release cached buffers and retry once
+        for (auto& kv : b->pool) (void)hipFree(kv.second);
+        b->pool.clear();
+        if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    }
+    return p;
+}
+
+// Estimated relative cost of a tile (LPT ordering: heavy tiles first).
+double tile_cost(const BatchGrid& g, int axis)
+{
+    return axis == 0 ? 60.0 * 4 * g.xrows : 256.0 * g.yH;
+}
+
+int rebuild_tiles(cfdart_batch* b)
+{
+    std::vector<std::pair<double, Tile>> v[2];
+    for (int s = 0; s < b->cap; ++s) {
+        if (!b->live[s]) continue;
+        const BatchGrid& g = b->h_grids[s];
+        auto& out = v[g.model == cfd::ALLAIRE ? 1 : 0];
+        int ntx = (g.nx + cfd::XTILE - 1) / cfd::XTILE, nry = (g.ny + 4 * g.xrows - 1) / (4 * g.xrows);
+        for (int a = 0; a < ntx; ++a)
+            for (int r = 0; r < nry; ++r) out.push_back({tile_cost(g, 0), Tile{s, 0, a, r}});
+        int nbx = (g.nx + cfd::YBLOCK - 1) / cfd::YBLOCK, nsy = (g.ny + g.yH - 1) / g.yH;
+        for (int a = 0; a < nbx; ++a)
+            for (int r = 0; r < nsy; ++r) out.push_back({tile_cost(g, 1), Tile{s, 1, a, r}});
+    }
+    b->ntiles = 0;
+    for (int m = 0; m < 2; ++m) {
+        std::stable_sort(v[m].begin(), v[m].end(),
+                         [](const std::pair<double, Tile>& x, const std::pair<double, Tile>& y) { return x.first > y.first; });
+        std::vector<Tile> t(v[m].size());
+        for (size_t i = 0; i < v[m].size(); ++i) t[i] = v[m][i].second;
+        if (t.size() > b->tile_cap[m]) {
+            BCHK(hipStreamSynchronize(b->st));
+            BCHK(hipStreamSynchronize(b->st2));
+            if (b->d_tiles[m]) BCHK(hipFree(b->d_tiles[m]));
+            b->tile_cap[m] = std::max(t.size(), 2 * b->tile_cap[m]);
+            BCHK(hipMalloc(&b->d_tiles[m], sizeof(Tile) * b->tile_cap[m]));
+            b->have_graph = false;  // the graph holds the tile pointers
+        }
+        if (!t.empty())
+            BCHK(hipMemcpyAsync(b->d_tiles[m], t.data(), sizeof(Tile) * t.size(), hipMemcpyHostToDevice, b->st));
+        b->ntile[m] = (int)t.size();
+        b->ntiles += (int)t.size();
+    }
+    hipLaunchKernelGGL(cfd::batch_reset_queue_kernel, dim3(1), dim3(64), 0, b->st, b->d_q, b->ntile[0], b->ntile[1]);
+    BCHK(hipGetLastError());
+    return 0;
+}
+
+// One step = both sweep phases of the Euler and the Allaire kernels (two concurrent branches),
+// joined before the per-grid loop bookkeeping.
+int capture_graph(cfdart_batch* b)
+{
+    if (b->have_graph) {
+        (void)hipGraphExecDestroy(b->exec);
+        b->have_graph = false;
+    }
+    for (int m = 0; m < 2; ++m)
+        if (!b->d_tiles[m]) {
+            b->tile_cap[m] = 1;
+            BCHK(hipMalloc(&b->d_tiles[m], sizeof(Tile)));
+        }
+    hipGraph_t graph;
+    BCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+    BCHK(hipEventRecord(b->fork_ev, b->st));
+    BCHK(hipStreamWaitEvent(b->st2, b->fork_ev, 0));
+    hipLaunchKernelGGL((cfd::batch_sweep_kernel<cfd::EULER, false>), dim3(b->nblocks[0]), dim3(256), 0, b->st,
+                       b->d_grids, b->d_tiles[0], b->d_q, 0);
+    hipLaunchKernelGGL((cfd::batch_sweep_kernel<cfd::EULER, true>), dim3(b->nblocks[0]), dim3(256), 0, b->st,
+                       b->d_grids, b->d_tiles[0], b->d_q, 1);
+    hipLaunchKernelGGL((cfd::batch_sweep_kernel<cfd::ALLAIRE, false>), dim3(b->nblocks[1]), dim3(256), 0, b->st2,
+                       b->d_grids, b->d_tiles[1], b->d_q, 0);
+    hipLaunchKernelGGL((cfd::batch_sweep_kernel<cfd::ALLAIRE, true>), dim3(b->nblocks[1]), dim3(256), 0, b->st2,
+                       b->d_grids, b->d_tiles[1], b->d_q, 1);
+    BCHK(hipEventRecord(b->join_ev, b->st2));
+    BCHK(hipStreamWaitEvent(b->st, b->join_ev, 0));
+    hipLaunchKernelGGL(cfd::batch_finalize_kernel, dim3((b->cap + 255) / 256), dim3(256), 0, b->st, b->d_grids,
+                       b->cap, b->d_q);
+    BCHK(hipStreamEndCapture(b->st, &graph));
+    hipError_t e = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return bfail(CFDART_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    b->have_graph = true;
+    return 0;
+}
+
+int check_slot(cfdart_batch* b, int slot)
+{
+    if (!b || slot < 0 || slot >= b->cap || !b->live[slot]) return bfail(CFDART_E_ARG, "bad batch slot");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cfdart_batch_create(int capacity, size_t mem_budget_bytes, cfdart_batch** out)
+{
+    if (!out || capacity < 1 || capacity > (1 << 20)) return bfail(CFDART_E_ARG, "bad capacity");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return bfail(CFDART_E_NODEV, "no HIP device");
+    cfdart_batch* b = new cfdart_batch();
+    b->cap = capacity;
+    b->budget = mem_budget_bytes;
+    b->used = 0;
+    b->h_grids.assign(capacity, BatchGrid{});
+    b->live.assign(capacity, 0);
+    b->bytes.assign(capacity, 0);
+    b->d_tiles[0] = b->d_tiles[1] = nullptr;
+    b->tile_cap[0] = b->tile_cap[1] = 0;
+    b->ntile[0] = b->ntile[1] = 0;
+    b->have_graph = false;
+    b->ntiles = 0;
+    b->d_snap = nullptr;
+    b->snap_cap = 0;
+    if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->st2, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->copy_st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->snap_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->join_ev, hipEventDisableTiming) != hipSuccess) {
+        delete b;
+        return bfail(CFDART_E_HIP, "stream creation failed");
+    }
+    if (hipMalloc(&b->d_grids, sizeof(BatchGrid) * capacity) != hipSuccess ||
+        hipMalloc(&b->d_q, sizeof(BatchQueue)) != hipSuccess ||
+        hipMalloc(&b->d_partials, sizeof(double) * 2 * 1024) != hipSuccess) {
+        cfdart_batch_destroy(b);
+        return bfail(CFDART_E_NOMEM, "batch tables");
+    }
+    (void)hipMemset(b->d_grids, 0, sizeof(BatchGrid) * capacity);
+    (void)hipMemset(b->d_q, 0, sizeof(BatchQueue));
+    // persistent grid: as many workgroups as can be resident at once
+    int dev = 0, cus = 0, per_cu[2] = {0, 0};
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[0], cfd::batch_sweep_kernel<cfd::EULER, true>, 256, 0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[1], cfd::batch_sweep_kernel<cfd::ALLAIRE, true>, 256, 0);
+    for (int m = 0; m < 2; ++m) b->nblocks[m] = std::max(1, cus * std::max(1, per_cu[m]));
+    if (hipDeviceSynchronize() != hipSuccess) {
+        cfdart_batch_destroy(b);
+        return bfail(CFDART_E_HIP, "device initialisation failed");
+    }
+    *out = b;
+    return 0;
+}
+
+void cfdart_batch_destroy(cfdart_batch* b)
+{
+    if (!b) return;
+    (void)hipStreamSynchronize(b->st);
+    (void)hipStreamSynchronize(b->st2);
+    (void)hipStreamSynchronize(b->copy_st);
+    for (int s = 0; s < b->cap; ++s)
+        if (b->live[s]) {
+            (void)hipFree(b->h_grids[s].buf[0]);
+            (void)hipFree(b->h_grids[s].buf[1]);
+        }
+    for (auto& kv : b->pool) (void)hipFree(kv.second);
+    if (b->have_graph) (void)hipGraphExecDestroy(b->exec);
+    for (int m = 0; m < 2; ++m)
+        if (b->d_tiles[m]) (void)hipFree(b->d_tiles[m]);
+    if (b->d_grids) (void)hipFree(b->d_grids);
+    if (b->d_q) (void)hipFree(b->d_q);
+    if (b->d_partials) (void)hipFree(b->d_partials);
+    if (b->d_snap) (void)hipFree(b->d_snap);
+    (void)hipEventDestroy(b->snap_ev);
+    (void)hipEventDestroy(b->fork_ev);
+    (void)hipEventDestroy(b->join_ev);
+    (void)hipStreamDestroy(b->st2);
+    (void)hipStreamDestroy(b->st);
+    (void)hipStreamDestroy(b->copy_st);
+    delete b;
+}
+
+int cfdart_batch_add(cfdart_batch* b, const cfdart_desc* d, const double* host, int splitting, int max_steps,
+                     double T, double cfl, int yH, int xrows, int* slot_out)
+{
+    if (!b || !d || !host || !slot_out) return bfail(CFDART_E_ARG, "null argument");
+    if (splitting != CFDART_NAIVE && splitting != CFDART_ALTERNATING)
+        return bfail(CFDART_E_ARG, "batched grids use naive or alternating splitting");
+    if (max_steps < 1 || !(cfl > 0.0)) return bfail(CFDART_E_ARG, "need max_steps >= 1 and cfl > 0");
+    // descriptor validation shared with cfdart_create
+    cfdart_grid* probe = nullptr;
+    (void)probe;
+    if (d->model != CFDART_EULER && d->model != CFDART_ALLAIRE) return bfail(CFDART_E_ARG, "bad model");
+    if (d->nx < 2 || d->ny < 2 || d->ng != 2 || !(d->dx > 0) || !(d->dy > 0)) return bfail(CFDART_E_ARG, "bad grid");
+    if (yH <= 0) yH = 32;
+    if (xrows <= 0) xrows = 4;
+    const int nc = d->model == CFDART_ALLAIRE ? 6 : 4;
+    const int pitch = (d->nx + 31) / 32 * 32;
+    const long long plane = (long long)pitch * d->ny;
+    const size_t one = round_up(sizeof(double) * (size_t)plane * nc, 2u << 20);
+    const size_t need = 2 * one;
+    if (b->budget && b->used + need > b->budget) return bfail(CFDART_E_NOMEM, "admission refused: memory budget");
+    int slot = -1;
+    for (int s = 0; s < b->cap; ++s)
+        if (!b->live[s]) { slot = s; break; }
+    if (slot < 0) return bfail(CFDART_E_NOMEM, "admission refused: batch is full");
+
+    BatchGrid g{};
+    g.buf[0] = pool_get(b, one);
+    g.buf[1] = g.buf[0] ? pool_get(b, one) : nullptr;
+    if (!g.buf[0] || !g.buf[1]) {
+        if (g.buf[0]) b->pool.insert({one, g.buf[0]});
+        return bfail(CFDART_E_NOMEM, "hipMalloc of the grid state failed");
+    }
+    g.plane = plane;
+    g.pitch = pitch;
+    g.nx = d->nx;
+    g.ny = d->ny;
+    g.model = d->model;
+    g.splitting = splitting;
+    for (int i = 0; i < 4; ++i) g.bc[i] = d->bc[i];
+    g.mat = cfd::make_material(d->gamma1, d->pinf1, d->gamma2, d->pinf2);
+    g.opt.limiter = d->limiter;
+    g.opt.wave_speeds = d->wave_speeds;
+    g.opt.z_lo = d->z_lo;
+    g.opt.z_hi = d->z_hi;
+    g.dx = d->dx;
+    g.dy = d->dy;
+    g.dx_half_col = d->colsweep_uses_dx ? d->dx : d->dy;
+    g.yH = yH;
+    g.xrows = xrows;
+    g.steps_left = max_steps;
+    g.live = 1;
+    std::memset(&g.ctrl, 0, sizeof(g.ctrl));
+    g.ctrl.T = T;
+    g.ctrl.cfl = cfl;
+
+    // interior of the host grid -> SoA buf[0]
+    std::vector<double> soa((size_t)plane * nc, 0.0);
+    const int W = d->nx + 2 * d->ng;
+    for (int k = 0; k < nc; ++k)
+        for (int y = 0; y < d->ny; ++y)
+            for (int x = 0; x < d->nx; ++x)
+                soa[(size_t)k * plane + (size_t)y * pitch + x] = host[((size_t)(y + d->ng) * W + (x + d->ng)) * nc + k];
+    BCHK(hipMemcpyAsync(g.buf[0], soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice, b->st));
+    BCHK(hipMemcpyAsync(b->d_grids + slot, &g, sizeof(BatchGrid), hipMemcpyHostToDevice, b->st));
+    // first dt: compute_dt's reduction on the new state into the grid's control block
+    cfd::GridView v;
+    v.cur = g.buf[0];
+    v.nxt = g.buf[1];
+    v.plane = plane;
+    v.pitch = pitch;
+    v.nx = g.nx;
+    v.ny = g.ny;
+    for (int i = 0; i < 4; ++i) v.bc[i] = g.bc[i];
+    v.mat = g.mat;
+    v.opt = g.opt;
+    v.dx = g.dx;
+    v.dy = g.dy;
+    v.dx_half_col = g.dx_half_col;
+    v.ctrl = &(b->d_grids + slot)->ctrl;
+    hipError_t e = cfd::launch_reduce_dt(v, g.model, b->d_partials, b->st);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(cfd::batch_admit_kernel, dim3(1), dim3(64), 0, b->st, b->d_grids + slot);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) return bfail(CFDART_E_HIP, std::string("admission kernels: ") + hipGetErrorString(e));
+    BCHK(hipStreamSynchronize(b->st));  // the staging vector goes out of scope
+    b->h_grids[slot] = g;
+    b->live[slot] = 1;
+    b->bytes[slot] = need;
+    b->used += need;
+    int rc = rebuild_tiles(b);
+    if (rc) return rc;
+    *slot_out = slot;
+    return 0;
+}
+
+int cfdart_batch_remove(cfdart_batch* b, int slot)
+{
+    int rc = check_slot(b, slot);
+    if (rc) return rc;
+    BatchGrid& g = b->h_grids[slot];
+    g.live = 0;
+    // mark dead on the device before the buffers can be reused
+    BCHK(hipMemcpyAsync(&(b->d_grids + slot)->live, &g.live, sizeof(int), hipMemcpyHostToDevice, b->st));
+    b->live[slot] = 0;
+    rc = rebuild_tiles(b);
+    if (rc) return rc;
+    BCHK(hipStreamSynchronize(b->st));
+    size_t one = b->bytes[slot] / 2;
+    b->pool.insert({one, g.buf[0]});
+    b->pool.insert({one, g.buf[1]});
+    b->used -= b->bytes[slot];
+    b->bytes[slot] = 0;
+    return 0;
+}
+
+int cfdart_batch_step(cfdart_batch* b, int nsteps)
+{
+    if (!b || nsteps < 0) return bfail(CFDART_E_ARG, "bad argument");
+    if (b->ntiles == 0) return 0;
+    if (!b->have_graph) {
+        int rc = capture_graph(b);
+        if (rc) return rc;
+    }
+    for (int i = 0; i < nsteps; ++i) BCHK(hipGraphLaunch(b->exec, b->st));
+    return 0;
+}
+
+int cfdart_batch_sync(cfdart_batch* b)
+{
+    if (!b) return bfail(CFDART_E_ARG, "null batch");
+    BCHK(hipStreamSynchronize(b->st));
+    BCHK(hipStreamSynchronize(b->st2));
+    BCHK(hipStreamSynchronize(b->copy_st));
+    return 0;
+}
+
+int cfdart_batch_status(cfdart_batch* b, int slot, double* t, int* n, int* active)
+{
+    int rc = check_slot(b, slot);
+    if (rc) return rc;
+    cfd::Ctrl c;
+    BCHK(hipMemcpyAsync(&c, &(b->d_grids + slot)->ctrl, sizeof(c), hipMemcpyDeviceToHost, b->st));
+    BCHK(hipStreamSynchronize(b->st));
+    if (t) *t = c.t;
+    if (n) *n = c.n;
+    if (active) *active = c.active;
+    if (c.error) return bfail(CFDART_E_UNPHYS, "an Allaire state failed the physical-state test");
+    return 0;
+}
+
+int cfdart_batch_stats(cfdart_batch* b, int slot, long long* counters)
+{
+    int rc = check_slot(b, slot);
+    if (rc) return rc;
+    if (!counters) return bfail(CFDART_E_ARG, "null counters");
+    cfd::Ctrl c;
+    BCHK(hipMemcpyAsync(&c, &(b->d_grids + slot)->ctrl, sizeof(c), hipMemcpyDeviceToHost, b->st));
+    BCHK(hipStreamSynchronize(b->st));
+    for (int i = 0; i < 4; ++i) counters[i] = (long long)c.cnt[i];
+    return 0;
+}
+
+int cfdart_batch_download(cfdart_batch* b, int slot, double* host)
+{
+    int rc = check_slot(b, slot);
+    if (rc) return rc;
+    if (!host) return bfail(CFDART_E_ARG, "null host buffer");
+    const BatchGrid& g = b->h_grids[slot];
+    const int nc = g.model == cfd::ALLAIRE ? 6 : 4, ng = 2, nx = g.nx, ny = g.ny, W = nx + 2 * ng;
+    std::vector<double> soa((size_t)g.plane * nc);
+    BCHK(hipStreamSynchronize(b->st));
+    BCHK(hipMemcpy(soa.data(), g.buf[0], sizeof(double) * soa.size(), hipMemcpyDeviceToHost));
+    auto at = [&](int i, int j, int k) -> double& { return host[((size_t)i * W + j) * nc + k]; };
+    for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x)
+            for (int k = 0; k < nc; ++k) at(y + ng, x + ng, k) = soa[(size_t)k * g.plane + (size_t)y * g.pitch + x];
+    const int mx = g.model == cfd::ALLAIRE ? 2 : 1, my = mx + 1;
+    for (int i = ng; i < ny + ng; ++i)
+        for (int j = 0; j < ng; ++j)
+            for (int k = 0; k < nc; ++k) {
+                at(i, j, k) = at(i, 2 * ng - j - 1, k) * ((k == mx && g.bc[0]) ? -1.0 : 1.0);
+                at(i, j + nx + ng, k) = at(i, nx + ng - 1 - j, k) * ((k == mx && g.bc[1]) ? -1.0 : 1.0);
+            }
+    for (int j = ng; j < nx + ng; ++j)
+        for (int i = 0; i < ng; ++i)
+            for (int k = 0; k < nc; ++k) {
+                at(i, j, k) = at(2 * ng - i - 1, j, k) * ((k == my && g.bc[2]) ? -1.0 : 1.0);
+                at(i + ny + ng, j, k) = at(ny + ng - 1 - i, j, k) * ((k == my && g.bc[3]) ? -1.0 : 1.0);
+            }
+    return 0;
+}
+
+int cfdart_batch_info(cfdart_batch* b, int* live, size_t* bytes, int* tiles)
+{
+    if (!b) return bfail(CFDART_E_ARG, "null batch");
+    int n = 0;
+    for (int s = 0; s < b->cap; ++s) n += b->live[s];
+    if (live) *live = n;
+    if (bytes) *bytes = b->used;
+    if (tiles) *tiles = b->ntiles;
+    return 0;
+}
+
+int cfdart_batch_frame(cfdart_batch* b, int slot, double* host_dst)
+{
+    int rc = check_slot(b, slot);
+    if (rc) return rc;
+    const BatchGrid& g = b->h_grids[slot];
+    const size_t n = (size_t)g.nx * g.ny * (g.model == cfd::ALLAIRE ? 6 : 4);
+    if (n > b->snap_cap) {
+        BCHK(hipStreamSynchronize(b->st));
+        BCHK(hipStreamSynchronize(b->copy_st));
+        if (b->d_snap) BCHK(hipFree(b->d_snap));
+        b->snap_cap = std::max(n, 2 * b->snap_cap);
+        BCHK(hipMalloc(&b->d_snap, sizeof(double) * b->snap_cap));
+    }
+    // the staging buffer must be free: the previous frame's copy has to be done
+    BCHK(hipStreamWaitEvent(b->st, b->snap_ev, 0));
+    hipLaunchKernelGGL(cfd::batch_snapshot_kernel, dim3(1024), dim3(256), 0, b->st, b->d_grids + slot, b->d_snap);
+    BCHK(hipGetLastError());
+    hipEvent_t ready;
+    BCHK(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    BCHK(hipEventRecord(ready, b->st));
+    BCHK(hipStreamWaitEvent(b->copy_st, ready, 0));
+    BCHK(hipMemcpyAsync(host_dst, b->d_snap, sizeof(double) * n, hipMemcpyDeviceToHost, b->copy_st));
+    BCHK(hipEventRecord(b->snap_ev, b->copy_st));
+    (void)hipEventDestroy(ready);
+    return 0;
+}
+
+int cfdart_batch_frames_done(cfdart_batch* b)
+{
+    if (!b) return bfail(CFDART_E_ARG, "null batch");
+    BCHK(hipStreamSynchronize(b->copy_st));
+    return 0;
+}
+
+void* cfdart_batch_stream(cfdart_batch* b) { return b ? (void*)b->st : nullptr; }
+
+void* cfdart_host_alloc(size_t bytes)
+{
+    void* p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
+    return p;
+}
+
+void cfdart_host_free(void* p)
+{
+    if (p) (void)hipHostFree(p);
+}
+
+int cfdart_time_tile(const cfdart_desc* d, int yH, int xrows, int reps, double* us_x, double* us_y)
+{
+    if (!d || !us_x || !us_y || reps < 1) return bfail(CFDART_E_ARG, "bad argument");
+    cfdart_grid* g = nullptr;
+    int rc = cfdart_create(d, &g);
+    if (rc) return rc;
+    const int nc = d->model == CFDART_ALLAIRE ? 6 : 4;
+    std::vector<double> h((size_t)(d->nx + 4) * (d->ny + 4) * nc);
+    double W[6], U[6];
+    if (d->model == CFDART_ALLAIRE) {
+        double z = 0.5 * (d->z_lo + d->z_hi);
+        double w[6] = {z * 1000.0, (1 - z) * 1.0, 10.0, 5.0, 1e5, z};
+        std::memcpy(W, w, sizeof(w));
+    } else {
+        double w[4] = {1.0, 0.3, 0.2, 1.0};
+        std::memcpy(W, w, sizeof(w));
+    }
+    cfdart_prim_to_cons(d, W, U);
+    for (size_t c = 0; c < h.size() / nc; ++c)
+        for (int k = 0; k < nc; ++k) h[c * nc + k] = U[k];
+    rc = cfdart_upload(g, h.data());
+    if (!rc) rc = cfdart_set_tile(g, yH, xrows);
+    if (!rc) rc = cfdart_set_profiling(g, 1);
+    double dt = 0.0;
+    if (!rc) rc = cfdart_compute_dt(g, 10, 0.0, INFINITY, 0.5, &dt, nullptr);
+    for (int i = 0; i < reps + 1 && !rc; ++i) {
+        if (i == 1) rc = cfdart_set_profiling(g, 1);  // first launches are warm-up
+        if (!rc) rc = cfdart_sweep(g, 0, dt);
+        if (!rc) rc = cfdart_sweep(g, 1, dt);
+    }
+    double ms[2];
+    long long nl[2];
+    if (!rc) rc = cfdart_profile(g, ms, nl);
+    if (!rc) {
+        *us_x = 1e3 * ms[0] / (double)std::max(1LL, nl[0]);
+        *us_y = 1e3 * ms[1] / (double)std::max(1LL, nl[1]);
+    }
+    if (rc) {
+        std::string m = cfdart_last_error();
+        cfdart_destroy(g);
+        return bfail(rc, m);
+    }
+    cfdart_destroy(g);
+    return 0;
+}
+
+}  // extern "C"

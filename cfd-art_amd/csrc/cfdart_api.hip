@@ -11,17 +11,6 @@
 
 #include "sweep_kernels.h"
 
-namespace cfd {
-hipError_t launch_sweep(const GridView& g, int model, int axis, bool last, double dt_arg,
-                        double dt_factor, int use_ctrl, int yH, hipStream_t st);
-hipError_t launch_reduce_dt(const GridView& g, int model, double* partials, hipStream_t st);
-hipError_t launch_init_dt(Ctrl* c, double dx, double dy, hipStream_t st);
-hipError_t launch_finalize(Ctrl* c, double dx, double dy, hipStream_t st);
-hipError_t launch_set_run(Ctrl* c, double T, double cfl, hipStream_t st);
-hipError_t launch_set_time(Ctrl* c, double t, int n, hipStream_t st);
-int reduce_blocks(const GridView& g);
-}  // namespace cfd
-
 struct cfdart_grid {
     cfdart_desc d;
     int nc;
@@ -33,7 +22,7 @@ struct cfdart_grid {
     double* partials;
     int nred;
     hipStream_t st;
-    int yH;
+    int yH, xrows;
     // optional per-kernel timing: an event before and after every sweep launch
     int profiling;
     std::vector<hipEvent_t> ev_pool;
@@ -140,7 +129,7 @@ int sweep_once(cfdart_grid* g, int axis, bool last, double dt_arg, double factor
         if (!e0 || !e1 || hipEventRecord(e0, g->st) != hipSuccess) return fail(CFDART_E_HIP, "event record failed");
         g->ev_marks.push_back({axis, (int)i0});
     }
-    hipError_t e = cfd::launch_sweep(v, g->d.model, axis, last, dt_arg, factor, use_ctrl, g->yH, g->st);
+    hipError_t e = cfd::launch_sweep(v, g->d.model, axis, last, dt_arg, factor, use_ctrl, g->yH, g->xrows, g->st);
     if (e != hipSuccess) return fail(CFDART_E_HIP, std::string("sweep launch: ") + hipGetErrorString(e));
     if (g->profiling && hipEventRecord(e1, g->st) != hipSuccess) return fail(CFDART_E_HIP, "event record failed");
     g->cur = 1 - g->cur;  // post_sweep: grid.swap(future_grid)
@@ -168,6 +157,10 @@ int step_plan(int splitting, int n, int* axes, double* factors)
 
 }  // namespace
 
+namespace cfd {
+void set_last_error(const std::string& msg) { g_err = msg; }
+}  // namespace cfd
+
 extern "C" {
 
 const char* cfdart_last_error(void) { return g_err.c_str(); }
@@ -193,6 +186,7 @@ int cfdart_create(const cfdart_desc* desc, cfdart_grid** out)
     g->plane = (long long)g->pitch * desc->ny;
     g->cur = 0;
     g->yH = 32;
+    g->xrows = 4;
     g->buf[0] = g->buf[1] = nullptr;
     g->ctrl = nullptr;
     g->partials = nullptr;
@@ -438,6 +432,15 @@ int cfdart_set_strip_rows(cfdart_grid* g, int rows)
 {
     if (!g || rows < 1 || rows > 4096) return fail(CFDART_E_ARG, "rows must be in [1, 4096]");
     g->yH = rows;
+    return 0;
+}
+
+int cfdart_set_tile(cfdart_grid* g, int y_strip_rows, int x_rows_per_wave)
+{
+    if (!g || y_strip_rows < 1 || y_strip_rows > 4096 || x_rows_per_wave < 1 || x_rows_per_wave > 256)
+        return fail(CFDART_E_ARG, "tile shape out of range");
+    g->yH = y_strip_rows;
+    g->xrows = x_rows_per_wave;
     return 0;
 }
 

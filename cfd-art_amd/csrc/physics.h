@@ -47,10 +47,13 @@ template <> struct Model<ALLAIRE> {
 };
 
 // Material constants. g1m1 = gamma1 - 1.0 and g1pinf1 = gamma1 * pinf1 are exactly the
-// sub-expressions the reference recomputes on every call.
+// sub-expressions the reference recomputes on every call. kmin and epsmin are lower bounds of
+// the Allaire mixture k and eps over z in [0, 1], used only by the conservative positivity
+// screens below (never in a value that is stored or compared bitwise).
 struct Material {
     double g1, pinf1, g2, pinf2;
     double g1m1, g2m1, g1pinf1;
+    double kmin, epsmin;
 };
 
 CFD_HD Material make_material(double g1, double pinf1, double g2, double pinf2)
@@ -58,6 +61,10 @@ CFD_HD Material make_material(double g1, double pinf1, double g2, double pinf2)
     Material m;
     m.g1 = g1; m.pinf1 = pinf1; m.g2 = g2; m.pinf2 = pinf2;
     m.g1m1 = g1 - 1.0; m.g2m1 = g2 - 1.0; m.g1pinf1 = g1 * pinf1;
+    double k1 = g1 * pinf1, k2 = g2 * pinf2;
+    m.kmin = k1 < k2 ? k1 : k2;
+    double e1 = 1.0 / (g1 - 1.0), e2 = 1.0 / (g2 - 1.0);
+    m.epsmin = (e1 < e2 ? e1 : e2) * (1.0 - 1e-12);
     return m;
 }
 
@@ -194,6 +201,48 @@ CFD_HD bool is_physical(const double* U, Model<ALLAIRE>*)
 }
 template <int M> CFD_HD bool is_physical(const double* U) { return is_physical(U, (Model<M>*)nullptr); }
 
+// Division-free screens for the physical-state tests. Each returns true only when the exact
+// test (with its divisions and roundings) is certain to return true; otherwise the caller runs
+// the exact test, so the boolean is always the reference's.
+//
+// Exact test on U = p2c(T): rho >= 0 and fl(E/rho) - fl(0.5(u''^2+v''^2)) >= 0, where
+// E = rho e_T + 0.5 rho (u^2+v^2), e_T = (p+K)/(G rho) and u'' = fl(fl(rho u)/rho). With every
+// value in the normal range each side carries a relative error of at most ~8 ulp, so the
+// computed difference is >= 0 whenever e_T >= 16 eps K_kin (K_kin = 0.5(u^2+v^2)), i.e. whenever
+// (p+K) >= 16 eps G rho K_kin. The screen demands (p+K) > 1e-12 G rho (u^2+v^2), a margin of
+// ~10^3, and bounds every magnitude so no intermediate leaves the normal range.
+CFD_HD bool prim_clearly_physical(double rho, double u, double v, double p, double K, double G)
+{
+    // K = gamma pinf (Euler) or a lower bound of the mixture k (Allaire); G = gamma - 1 (Euler)
+    // or 1/epsmin (Allaire). The relative margin on p + K keeps its rounding (and that of the
+    // mixture k) far below p + K itself.
+    double w = p + K;
+    double q = u * u + v * v;
+    return rho > 1e-100 && rho < 1e100 && q < 1e100 && w > 1e-100 * rho && w < 1e100 * rho &&
+           w > 1e-9 * (fabs(p) + K) && w > 1e-12 * G * rho * q;
+}
+
+// Exact test on a conserved state with density rho, energy E and momenta mu, mv:
+// fl(E/rho) - fl(0.5((mu/rho)^2 + (mv/rho)^2)) >= 0 holds whenever
+// E rho - 0.5(mu^2 + mv^2) >= 6 eps E rho; the computed difference q has an absolute error
+// below 6 eps E rho, so q >= 1e-12 fl(E rho) is sufficient.
+CFD_HD bool cons_clearly_physical(double rho, double mu, double mv, double E)
+{
+    double er = E * rho;
+    double q = er - 0.5 * (mu * mu + mv * mv);
+    return rho > 1e-100 && rho < 1e100 && er < 1e200 && fabs(mu) < 1e100 && fabs(mv) < 1e100 &&
+           er > 1e-200 && q >= 1e-12 * er;
+}
+
+// Allaire post-update test (after snapping): a division-free screen, then the exact test.
+CFD_HD bool allaire_physical_screened(const double* U)
+{
+    if (U[0] >= 0.0 && U[1] >= 0.0 && U[5] >= 0.0 && U[5] <= 1.0 &&
+        cons_clearly_physical(U[0] + U[1], U[2], U[3], U[4]))
+        return true;
+    return is_physical<ALLAIRE>(U);
+}
+
 // ---------------------------------------------------------------- limiter
 
 CFD_HD double limit(int limiter, double a, double b)
@@ -244,13 +293,48 @@ CFD_HD void side_from_cons(const Material& m, Side<ALLAIRE>& s)
     s.ok = U[0] >= 0.0 && U[1] >= 0.0 && e >= 0.0 && z >= 0.0 && z <= 1.0;
 }
 
+// HLLC inputs of a conserved Allaire state whose mixture coefficients are already known.
+CFD_HD void side_from_cons_mix(const Mix& x, Side<ALLAIRE>& s)
+{
+    const double* U = s.U;
+    double rho = U[0] + U[1];
+    double u = U[2] / rho;
+    double v = U[3] / rho;
+    double e = U[4] / rho - 0.5 * (u * u + v * v);
+    double z = U[5];
+    double p = MixEos::pressure(x, rho, e);
+    s.rho = rho; s.u = u; s.p = p;
+    s.c = MixEos::soundspeed(x, rho, p);
+    s.ok = U[0] >= 0.0 && U[1] >= 0.0 && e >= 0.0 && z >= 0.0 && z <= 1.0;
+}
+
 // Evolved state from its primitive vector T (step M4 result): p2c, then the M5 test; the
 // Allaire test snaps the state in place and the snapped state is what HLLC receives.
+CFD_HD void side_from_prim(const Material& m, const double* T, Side<EULER>& s)
+{
+    p2c<EULER>(m, T, s.U);
+    side_from_cons(m, s);
+}
+CFD_HD void side_from_prim(const Material& m, const double* T, Side<ALLAIRE>& s)
+{
+    // p2c with the mixture of z = T[5]; the same coefficients serve HLLC unless snapping
+    // changes z (then they are recomputed from the snapped z, as the reference does)
+    double rho = T[0] + T[1];
+    Mix x = mixture(m, T[5]);
+    double e = MixEos::specific_ie(x, rho, T[4]);
+    s.U[0] = T[0];
+    s.U[1] = T[1];
+    s.U[2] = rho * T[2];
+    s.U[3] = rho * T[3];
+    s.U[4] = rho * e + 0.5 * rho * (T[2] * T[2] + T[3] * T[3]);
+    s.U[5] = T[5];
+    snap(s.U);
+    if (s.U[5] != T[5]) x = mixture(m, s.U[5]);
+    side_from_cons_mix(x, s);
+}
 template <int M> CFD_HD void side_from_prim(const Material& m, const double* T, Side<M>& s)
 {
-    p2c<M>(m, T, s.U);
-    if (M == ALLAIRE) snap(s.U);
-    side_from_cons(m, s);
+    side_from_prim(m, T, s);
 }
 
 // The first-order fallback state (M5 failed): the raw stencil cell, unsnapped.
@@ -299,12 +383,27 @@ CFD_HD void jac_times(const double* W, double c, double s, const double* d, doub
 }
 
 // M3 test of one extrapolated state (Allaire: the snapping overload on a temporary).
+CFD_HD bool extrap_physical(const Material& m, const double* T, Model<EULER>*)
+{
+    if (prim_clearly_physical(T[0], T[1], T[2], T[3], m.g1pinf1, m.g1m1)) return true;
+    double U[4];
+    p2c<EULER>(m, T, U);
+    return is_physical<EULER>(U);
+}
+CFD_HD bool extrap_physical(const Material& m, const double* T, Model<ALLAIRE>*)
+{
+    // no snapping can occur when both partial densities and z are away from 0 (and z from 1)
+    if (T[0] >= 1e-12 && T[1] >= 1e-12 && T[5] >= 1e-12 && T[5] <= 1.0 - 1e-12 &&
+        prim_clearly_physical(T[0] + T[1], T[2], T[3], T[4], m.kmin, 1.0 / m.epsmin))
+        return true;
+    double U[6];
+    p2c<ALLAIRE>(m, T, U);
+    snap(U);
+    return is_physical<ALLAIRE>(U);
+}
 template <int M> CFD_HD bool extrap_physical(const Material& m, const double* T)
 {
-    double U[Model<M>::NC];
-    p2c<M>(m, T, U);
-    if (M == ALLAIRE) snap(U);
-    return is_physical<M>(U);
+    return extrap_physical(m, T, (Model<M>*)nullptr);
 }
 
 // Cell stage from the cell's primitives W (with c) and its two neighbours' primitives.
@@ -445,6 +544,7 @@ CFD_HD int face_stage(const Material& m, const SchemeOpts& o, double s,
 // Euler repair (E/sourcecode/euler_rowsweep.cpp:47-82): 'old' and 'fut' in the unrotated frame.
 CFD_HD bool euler_repair(const Material& m, const double* old, double* fut)
 {
+    if (cons_clearly_physical(fut[0], fut[1], fut[2], fut[3])) return false;
     double rho = fut[0];
     double u = fut[1] / rho;
     double v = fut[2] / rho;

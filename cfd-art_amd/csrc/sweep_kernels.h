@@ -1,6 +1,8 @@
 // Device data structures shared by the sweep kernels and the C-ABI layer.
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include "physics.h"
 
 namespace cfd {
@@ -36,7 +38,16 @@ struct GridView {
 
 // Sweep-kernel launch geometry.
 constexpr int XTILE = 60;         // output columns per wavefront in the x sweep (64 lanes - 4 halo)
-constexpr int XROWS_PER_WAVE = 4; // rows each x-sweep wavefront processes
 constexpr int YBLOCK = 256;       // threads (columns) per y-sweep block
+
+// Host-side launchers (sweep_kernels.hip).
+hipError_t launch_sweep(const GridView& g, int model, int axis, bool last, double dt_arg,
+                        double dt_factor, int use_ctrl, int yH, int xrows, hipStream_t st);
+hipError_t launch_reduce_dt(const GridView& g, int model, double* partials, hipStream_t st);
+hipError_t launch_init_dt(Ctrl* c, double dx, double dy, hipStream_t st);
+hipError_t launch_finalize(Ctrl* c, double dx, double dy, hipStream_t st);
+hipError_t launch_set_run(Ctrl* c, double T, double cfl, hipStream_t st);
+hipError_t launch_set_time(Ctrl* c, double t, int n, hipStream_t st);
+int reduce_blocks(const GridView& g);
 
 }  // namespace cfd

@@ -10,320 +10,43 @@
 // face fluxes, the conservative update, the Euler repair / Allaire z-update and snapping, and,
 // in the last sweep of a step, the per-cell wave speeds of compute_dt
 // (E/sourcecode/euler_timestep.cpp:14-25) reduced to one atomic max per wavefront.
-#include "sweep_kernels.h"
+#include "sweep_device.h"
 
 #include <hip/hip_runtime.h>
 
 namespace cfd {
 
-// ------------------------------------------------------------------ lane exchange
-
-__device__ __forceinline__ double lane_from_left(double x)  // lane i receives lane i-1
-{
-    int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x138, 0xf, 0xf, false);
-    int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x138, 0xf, 0xf, false);
-    return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ double lane_from_right(double x)  // lane i receives lane i+1
-{
-    int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x130, 0xf, 0xf, false);
-    int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x130, 0xf, 0xf, false);
-    return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ int lane_from_left_i(int x)
-{
-    return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false);
-}
-
-template <int M> __device__ __forceinline__ void side_from_left(const Side<M>& s, Side<M>& o)
-{
-#pragma unroll
-    for (int k = 0; k < Model<M>::NC; ++k) o.U[k] = lane_from_left(s.U[k]);
-    o.rho = lane_from_left(s.rho);
-    o.u = lane_from_left(s.u);
-    o.p = lane_from_left(s.p);
-    o.c = lane_from_left(s.c);
-    o.ok = lane_from_left_i(s.ok ? 1 : 0) != 0;
-}
-
-// Ghost index of the reference's pre_sweep: interior index i of an axis of n cells, with
-// i in [-2, n+2) mapped to its mirror. Indices further out (dead lanes) are clamped in range.
-__device__ __forceinline__ int mirror(int i, int n)
-{
-    int r = i < 0 ? -1 - i : (i >= n ? 2 * n - 1 - i : i);
-    return r < 0 ? 0 : (r >= n ? n - 1 : r);
-}
-
-__device__ __forceinline__ double wave_max(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = smax(v, __shfl_xor(v, o));
-    return v;
-}
-__device__ __forceinline__ int wave_sum(int v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
-
-struct Tally {
-    double su = 0.0, sv = 0.0;  // running max of |u|+c, |v|+c (NaN never enters: smax keeps the old value)
-    int slope = 0, evolve = 0, repair = 0, bad = 0;
-    __device__ void flush(Ctrl* c, bool last)
-    {
-        int lane = threadIdx.x & 63;
-        if (last) {
-            double mu = wave_max(su), mv = wave_max(sv);
-            if (lane == 0) {
-                atomicMax(&c->maxu, (unsigned long long)__double_as_longlong(mu));
-                atomicMax(&c->maxv, (unsigned long long)__double_as_longlong(mv));
-            }
-        }
-        int a = wave_sum(slope), b = wave_sum(evolve), d = wave_sum(repair), e = wave_sum(bad);
-        if (lane == 0) {
-            if (a) atomicAdd(&c->cnt[0], (unsigned long long)a);
-            if (b) atomicAdd(&c->cnt[1], (unsigned long long)b);
-            if (d) atomicAdd(&c->cnt[2], (unsigned long long)d);
-            if (e) { atomicAdd(&c->cnt[3], (unsigned long long)e); atomicOr(&c->error, 1); }
-        }
-    }
-};
-
-__device__ __forceinline__ bool sweep_dt(const GridView& g, double dt_arg, double dt_factor,
-                                         int use_ctrl, double& dt)
-{
-    if (!use_ctrl) { dt = dt_arg; return true; }
-    if (!g.ctrl->active) return false;
-    dt = dt_factor * g.ctrl->dt;
-    return true;
-}
-
-// Conservative update + model-specific post-processing of one cell (row-sweep frame or
-// rotated column frame): E/sourcecode/euler_rowsweep.cpp:36-39,
-// A/sourcecode/allaire_diffuse_rowsweep.cpp:36-50.
-template <int M>
-__device__ __forceinline__ void update_cell(const GridView& g, double r, const double* U,
-                                           const double* FL, const double* FR, double usL, double usR,
-                                           double zsL, double zsR, double* Un, bool& bad)
-{
-    constexpr int NC = Model<M>::NC;
-#pragma unroll
-    for (int k = 0; k < NC; ++k) Un[k] = U[k] + r * (FL[k] - FR[k]);
-    bad = false;
-    if (M == ALLAIRE) {
-        Un[5] = zupdate(r, U[5], usL, usR, zsL, zsR, g.opt.z_lo, g.opt.z_hi);
-        snap(Un);
-        bad = !is_physical<M>(Un);
-    }
-}
-
-// ------------------------------------------------------------------ x sweep
-
 template <int M, bool LAST>
-__global__ __launch_bounds__(256) void sweep_x_kernel(GridView g, double dt_arg, double dt_factor, int use_ctrl)
+__global__ __launch_bounds__(256) void sweep_x_kernel(GridView g, double dt_arg, double dt_factor, int use_ctrl,
+                                                     int xrows)
 {
-    constexpr int NC = Model<M>::NC;
-    constexpr int MN = Model<M>::MN;
-    const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int x = blockIdx.x * XTILE - 2 + lane;
-    const bool store = lane >= 2 && lane < 62 && x < g.nx;
-    const int y0 = (blockIdx.y * 4 + wave) * XROWS_PER_WAVE;
+    const int row0 = (blockIdx.y * 4 + wave) * xrows;
     double dt;
     if (!sweep_dt(g, dt_arg, dt_factor, use_ctrl, dt)) {
-        // the time loop has ended (t >= T): the sweep is the identity so the buffer swap holds
-        if (store)
-            for (int y = y0; y < min(y0 + XROWS_PER_WAVE, g.ny); ++y)
-                for (int k = 0; k < NC; ++k)
-                    g.nxt[k * g.plane + (long long)y * g.pitch + x] = g.cur[k * g.plane + (long long)y * g.pitch + x];
+        const int lane = threadIdx.x & 63, x = blockIdx.x * XTILE - 2 + lane;
+        copy_rows<M>(g, x, lane >= 2 && lane < 62 && x < g.nx, row0, min(row0 + xrows, g.ny));
         return;
     }
-    const double s = 0.5 * (dt / g.dx);
-    const double r = dt / g.dx;
-    const int xs = mirror(x, g.nx);
-    const bool neg = (x < 0 && g.bc[0]) || (x >= g.nx && g.bc[1]);
-    const bool face_counted = (lane >= 2 && lane < 62 && x <= g.nx) || (lane == 62 && x == g.nx);
     Tally tl;
-
-    for (int it = 0; it < XROWS_PER_WAVE; ++it) {
-        const int y = y0 + it;
-        if (y >= g.ny) break;
-        double U[NC];
-        const double* base = g.cur + (long long)y * g.pitch + xs;
-#pragma unroll
-        for (int k = 0; k < NC; ++k) U[k] = base[k * g.plane];
-        if (neg) U[MN] = -U[MN];
-
-        Prim<M> P;
-        P.from_cons(g.mat, U);
-        double Wm[NC], Wp[NC];
-#pragma unroll
-        for (int k = 0; k < NC; ++k) { Wm[k] = lane_from_left(P.W[k]); Wp[k] = lane_from_right(P.W[k]); }
-        CellStage<M> cs;
-        cell_stage<M>(g.mat, g.opt.limiter, s, P, Wm, Wp, cs);
-
-        // left neighbour's data for this lane's left face
-        Prim<M> Pa;
-#pragma unroll
-        for (int k = 0; k < NC; ++k) Pa.W[k] = Wm[k];
-        Pa.c = lane_from_left(P.c);
-        double Ua[NC];
-#pragma unroll
-        for (int k = 0; k < NC; ++k) Ua[k] = lane_from_left(U[k]);
-        CellStage<M> a;
-        a.fR = lane_from_left_i(cs.fR ? 1 : 0) != 0;
-        side_from_left<M>(cs.R, a.R);
-
-        FaceOut<M> fo;
-        int fl = face_stage<M>(g.mat, g.opt, s, Pa, Ua, a, P, U, cs, fo);
-
-        double Fr[NC];
-#pragma unroll
-        for (int k = 0; k < NC; ++k) Fr[k] = lane_from_right(fo.F[k]);
-        double usr = lane_from_right(fo.ustar);
-        double zsr = M == ALLAIRE ? lane_from_right(fo.zstar) : 0.0;
-
-        double Un[NC];
-        bool bad;
-        update_cell<M>(g, r, U, fo.F, Fr, fo.ustar, usr, fo.zstar, zsr, Un, bad);
-        bool rep = false;
-        if (M == EULER) rep = euler_repair(g.mat, U, Un);
-
-        if (face_counted) { tl.slope += fl & 1; tl.evolve += (fl >> 1) & 1; }
-        if (store) {
-            double* out = g.nxt + (long long)y * g.pitch + x;
-#pragma unroll
-            for (int k = 0; k < NC; ++k) __builtin_nontemporal_store(Un[k], out + k * g.plane);
-            tl.repair += rep;
-            tl.bad += bad;
-            if (LAST) {
-                double su, sv;
-                wave_speeds(g.mat, Un, su, sv, (Model<M>*)nullptr);
-                tl.su = smax(tl.su, su);
-                tl.sv = smax(tl.sv, sv);
-            }
-        }
-    }
+    x_tile<M, LAST>(g, dt, blockIdx.x, row0, xrows, tl);
     tl.flush(g.ctrl, LAST);
-}
-
-// ------------------------------------------------------------------ y sweep
-
-template <int M>
-__device__ __forceinline__ void load_rot(const GridView& g, int y, int xs, double* U)
-{
-    constexpr int NC = Model<M>::NC;
-    constexpr int MX = Model<M>::MN, MY = Model<M>::MT;
-    const int ys = mirror(y, g.ny);
-    const double* b = g.cur + (long long)ys * g.pitch + xs;
-    double raw[NC];
-#pragma unroll
-    for (int k = 0; k < NC; ++k) raw[k] = b[k * g.plane];
-    // ghost rows of a reflective bottom/top wall negate rho*v (E/sourcecode/euler_ppsweep.cpp:79,105)
-    if ((y < 0 && g.bc[2]) || (y >= g.ny && g.bc[3])) raw[MY] = -raw[MY];
-    // rotation (m_x, m_y) -> (m_y, -m_x) of E/sourcecode/euler_colsweep.cpp:118-124
-#pragma unroll
-    for (int k = 0; k < NC; ++k) U[k] = raw[k];
-    U[MX] = raw[MY];
-    U[MY] = -raw[MX];
 }
 
 template <int M, bool LAST>
 __global__ __launch_bounds__(YBLOCK) void sweep_y_kernel(GridView g, double dt_arg, double dt_factor,
                                                         int use_ctrl, int H)
 {
-    constexpr int NC = Model<M>::NC;
-    constexpr int MX = Model<M>::MN, MY = Model<M>::MT;
-    const int x = blockIdx.x * YBLOCK + threadIdx.x;
-    const bool valid = x < g.nx;
     const int y0 = blockIdx.y * H;
     const int y1 = min(y0 + H, g.ny);
     double dt;
     if (!sweep_dt(g, dt_arg, dt_factor, use_ctrl, dt)) {
-        if (valid)
-            for (int y = y0; y < y1; ++y)
-                for (int k = 0; k < NC; ++k)
-                    g.nxt[k * g.plane + (long long)y * g.pitch + x] = g.cur[k * g.plane + (long long)y * g.pitch + x];
+        const int x = blockIdx.x * YBLOCK + threadIdx.x;
+        copy_rows<M>(g, x, x < g.nx, y0, y1);
         return;
     }
-    const double s = 0.5 * (dt / g.dx_half_col);
-    const double r = dt / g.dy;
-    const int xs = valid ? x : g.nx - 1;
     Tally tl;
-
-    double Um[NC], U0[NC], Up[NC];
-    Prim<M> Pm, P0, Pp;
-    CellStage<M> a;   // cell c-1 (its R side and fR are used)
-    FaceOut<M> Fm;    // face c-1
-
-    load_rot<M>(g, y0 - 2, xs, Um);
-    Pm.from_cons(g.mat, Um);
-    load_rot<M>(g, y0 - 1, xs, U0);
-    P0.from_cons(g.mat, U0);
-    load_rot<M>(g, y0, xs, Up);
-    Pp.from_cons(g.mat, Up);
-    cell_stage<M>(g.mat, g.opt.limiter, s, P0, Pm.W, Pp.W, a);  // cell y0-1
-    // shift: c-1 <- y0-1, c <- y0
-#pragma unroll
-    for (int k = 0; k < NC; ++k) { Um[k] = U0[k]; U0[k] = Up[k]; }
-    Pm = P0;
-    P0 = Pp;
-
-    double Unext[NC];
-    load_rot<M>(g, y0 + 1, xs, Unext);
-    for (int c = y0; c <= y1; ++c) {
-        // window: Um = row c-1, U0 = row c, Unext = row c+1
-#pragma unroll
-        for (int k = 0; k < NC; ++k) Up[k] = Unext[k];
-        if (c + 2 <= y1 + 1) load_rot<M>(g, c + 2, xs, Unext);
-        Pp.from_cons(g.mat, Up);
-        CellStage<M> cs;
-        cell_stage<M>(g.mat, g.opt.limiter, s, P0, Pm.W, Pp.W, cs);
-        FaceOut<M> fo;
-        int fl = face_stage<M>(g.mat, g.opt, s, Pm, Um, a, P0, U0, cs, fo);
-        if (valid && (c < y1 || c == g.ny)) { tl.slope += fl & 1; tl.evolve += (fl >> 1) & 1; }
-        if (c > y0) {
-            // update cell c-1 from faces c-1 (Fm) and c (fo)
-            double Un[NC];
-            bool bad;
-            update_cell<M>(g, r, Um, Fm.F, fo.F, Fm.ustar, fo.ustar, Fm.zstar, fo.zstar, Un, bad);
-            // rotate back (m_x, m_y) -> (-m_y, m_x): E/sourcecode/euler_colsweep.cpp:163-168
-            double t = Un[MX];
-            Un[MX] = -Un[MY];
-            Un[MY] = t;
-            bool rep = false;
-            if (M == EULER) {
-                double old[NC];
-#pragma unroll
-                for (int k = 0; k < NC; ++k) old[k] = Um[k];
-                old[MX] = -Um[MY];
-                old[MY] = Um[MX];
-                rep = euler_repair(g.mat, old, Un);
-            }
-            if (valid) {
-                double* out = g.nxt + (long long)(c - 1) * g.pitch + x;
-#pragma unroll
-                for (int k = 0; k < NC; ++k) __builtin_nontemporal_store(Un[k], out + k * g.plane);
-                tl.repair += rep;
-                tl.bad += bad;
-                if (LAST) {
-                    double su, sv;
-                    wave_speeds(g.mat, Un, su, sv, (Model<M>*)nullptr);
-                    tl.su = smax(tl.su, su);
-                    tl.sv = smax(tl.sv, sv);
-                }
-            }
-        }
-        // shift the window
-#pragma unroll
-        for (int k = 0; k < NC; ++k) { Um[k] = U0[k]; U0[k] = Up[k]; }
-        Pm = P0;
-        P0 = Pp;
-        a.fR = cs.fR;
-        a.R = cs.R;
-        Fm = fo;
-    }
+    y_tile<M, LAST>(g, dt, blockIdx.x, y0, y1, tl);
     tl.flush(g.ctrl, LAST);
 }
 
@@ -367,18 +90,6 @@ __global__ __launch_bounds__(256) void reduce_dt_kernel(GridView g, double* part
         partials[2 * blockIdx.x] = ((sm[0][0] + sm[0][1]) + (sm[0][2] + sm[0][3]));
         partials[2 * blockIdx.x + 1] = ((sm[1][0] + sm[1][1]) + (sm[1][2] + sm[1][3]));
     }
-}
-
-// dt from the reduced maxima (E/sourcecode/euler_timestep.cpp:35-43).
-__device__ __forceinline__ double dt_from_maxima(const Ctrl* c, int n, double t, double dx, double dy)
-{
-    double mu = __longlong_as_double((long long)c->maxu);
-    double mv = __longlong_as_double((long long)c->maxv);
-    double CFL = c->cfl;
-    if (n < 5) CFL = smin(CFL, 0.2);
-    double dt = CFL * smin(dx / mu, dy / mv);
-    if (t + dt > c->T) dt = c->T - t;
-    return dt;
 }
 
 // Start of the time loop: dt for (n, t) from a fresh reduction, and the 'while (t < T)' test.
@@ -431,10 +142,10 @@ hipError_t launch_set_time(Ctrl* c, double t, int n, hipStream_t st)
 }
 
 template <int M, bool LAST>
-static hipError_t launch_x(const GridView& g, double dt_arg, double f, int use_ctrl, hipStream_t st)
+static hipError_t launch_x(const GridView& g, double dt_arg, double f, int use_ctrl, int xrows, hipStream_t st)
 {
-    dim3 grid((g.nx + XTILE - 1) / XTILE, (g.ny + 4 * XROWS_PER_WAVE - 1) / (4 * XROWS_PER_WAVE));
-    hipLaunchKernelGGL((sweep_x_kernel<M, LAST>), grid, dim3(256), 0, st, g, dt_arg, f, use_ctrl);
+    dim3 grid((g.nx + XTILE - 1) / XTILE, (g.ny + 4 * xrows - 1) / (4 * xrows));
+    hipLaunchKernelGGL((sweep_x_kernel<M, LAST>), grid, dim3(256), 0, st, g, dt_arg, f, use_ctrl, xrows);
     return hipGetLastError();
 }
 template <int M, bool LAST>
@@ -446,16 +157,16 @@ static hipError_t launch_y(const GridView& g, double dt_arg, double f, int use_c
 }
 
 hipError_t launch_sweep(const GridView& g, int model, int axis, bool last, double dt_arg,
-                        double dt_factor, int use_ctrl, int yH, hipStream_t st)
+                        double dt_factor, int use_ctrl, int yH, int xrows, hipStream_t st)
 {
     if (model == EULER) {
-        if (axis == 0) return last ? launch_x<EULER, true>(g, dt_arg, dt_factor, use_ctrl, st)
-                                   : launch_x<EULER, false>(g, dt_arg, dt_factor, use_ctrl, st);
+        if (axis == 0) return last ? launch_x<EULER, true>(g, dt_arg, dt_factor, use_ctrl, xrows, st)
+                                   : launch_x<EULER, false>(g, dt_arg, dt_factor, use_ctrl, xrows, st);
         return last ? launch_y<EULER, true>(g, dt_arg, dt_factor, use_ctrl, yH, st)
                     : launch_y<EULER, false>(g, dt_arg, dt_factor, use_ctrl, yH, st);
     }
-    if (axis == 0) return last ? launch_x<ALLAIRE, true>(g, dt_arg, dt_factor, use_ctrl, st)
-                               : launch_x<ALLAIRE, false>(g, dt_arg, dt_factor, use_ctrl, st);
+    if (axis == 0) return last ? launch_x<ALLAIRE, true>(g, dt_arg, dt_factor, use_ctrl, xrows, st)
+                               : launch_x<ALLAIRE, false>(g, dt_arg, dt_factor, use_ctrl, xrows, st);
     return last ? launch_y<ALLAIRE, true>(g, dt_arg, dt_factor, use_ctrl, yH, st)
                 : launch_y<ALLAIRE, false>(g, dt_arg, dt_factor, use_ctrl, yH, st);
 }

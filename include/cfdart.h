@@ -111,6 +111,8 @@ int cfdart_set_profiling(cfdart_grid* g, int enable);
 int cfdart_profile(cfdart_grid* g, double* ms_total /* [2] */, long long* launches /* [2] */);
 /* Tuning: rows per y-sweep strip (default 32). */
 int cfdart_set_strip_rows(cfdart_grid* g, int rows);
+/* Tile shape: rows per y-sweep strip (default 32) and rows per x-sweep wavefront (default 4). */
+int cfdart_set_tile(cfdart_grid* g, int y_strip_rows, int x_rows_per_wave);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,76 @@
+/*
+ * cfdart batch: many independent grids (Euler and Allaire, uneven sizes) advancing together.
+ *
+ * No counterpart in the reference, which runs one grid per process
+ * (euler_equations/euler_simulation_code/sourcecode/main.cpp:17-49). Each grid in a batch
+ * follows exactly the reference's time loop (simulation.hpp:48-61) with its own CFL time step,
+ * step budget and end time; its results are bitwise those of the same grid run alone through
+ * cfdart.h. Per step, one persistent kernel launch per sweep processes the tiles of every live
+ * grid from a device work-list, and the step (sweep, sweep, loop bookkeeping) is captured once
+ * in a HIP graph and replayed. Grids join and leave between steps by editing device-resident
+ * tables, without re-capturing the graph.
+ *
+ * Splitting schemes: CFDART_NAIVE and CFDART_ALTERNATING (two sweeps per step).
+ * Conventions as in cfdart.h: 0 or a negative CFDART_E_* code; one host thread per batch.
+ */
+#ifndef CFDART_BATCH_H
+#define CFDART_BATCH_H
+
+#include <stddef.h>
+
+#include "cfdart.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cfdart_batch cfdart_batch;
+
+/* capacity: maximum number of live grids; mem_budget_bytes: admission budget for grid state
+ * (two FP64 buffers per grid), 0 = no limit. */
+int cfdart_batch_create(int capacity, size_t mem_budget_bytes, cfdart_batch** out);
+void cfdart_batch_destroy(cfdart_batch* b);
+
+/* Admits a grid: uploads host_state (format of cfdart.h), computes its first dt and adds its
+ * tiles to the work-list. It then runs until it has taken max_steps steps or reached t >= T.
+ * y_strip_rows / x_rows_per_wave: tile shape (0 = default). Returns CFDART_E_NOMEM when the
+ * memory budget or the capacity would be exceeded (nothing is changed). slot_out receives the
+ * grid's slot id. */
+int cfdart_batch_add(cfdart_batch* b, const cfdart_desc* desc, const double* host_state, int splitting,
+                     int max_steps, double T, double cfl, int y_strip_rows, int x_rows_per_wave,
+                     int* slot_out);
+/* Removes a grid (finished or not) and frees its device memory for reuse. */
+int cfdart_batch_remove(cfdart_batch* b, int slot);
+/* Advances every live grid by nsteps steps (grids that have finished stay unchanged). */
+int cfdart_batch_step(cfdart_batch* b, int nsteps);
+/* Copies a grid's current state to the host (format of cfdart.h, ghost frame filled). */
+int cfdart_batch_download(cfdart_batch* b, int slot, double* host_state);
+/* Loop state of a grid: t, steps taken, and whether it is still running; synchronous. */
+int cfdart_batch_status(cfdart_batch* b, int slot, double* t, int* n, int* active);
+/* Fallback counters of a grid (see CFDART_CNT_*), 4 values; synchronous. */
+int cfdart_batch_stats(cfdart_batch* b, int slot, long long* counters);
+int cfdart_batch_sync(cfdart_batch* b);
+/* Live grids, device bytes held by grid state, tiles in the work-list. */
+int cfdart_batch_info(cfdart_batch* b, int* live, size_t* bytes, int* tiles);
+/* Frame snapshot: device-to-device copy of a grid's state into a staging buffer on the batch
+ * stream, then an asynchronous copy to host_dst (pinned memory, at least the state's interior
+ * size: ncomp * nx * ny doubles, SoA planes) on the copy stream. Completion: cfdart_batch_sync
+ * or cfdart_batch_frames_done. */
+int cfdart_batch_frame(cfdart_batch* b, int slot, double* host_dst);
+int cfdart_batch_frames_done(cfdart_batch* b);
+void* cfdart_batch_stream(cfdart_batch* b);
+
+/* Pinned host memory for frames (hipHostMalloc), freed with cfdart_host_free. */
+void* cfdart_host_alloc(size_t bytes);
+void cfdart_host_free(void* p);
+
+/* Autotuner measurement: times one sweep of each axis for a grid of this description with a
+ * given tile shape (synthetic uniform state), returning the mean kernel time in microseconds
+ * over 'reps' launches per axis. */
+int cfdart_time_tile(const cfdart_desc* desc, int y_strip_rows, int x_rows_per_wave, int reps,
+                     double* us_x, double* us_y);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

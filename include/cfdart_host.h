@@ -42,6 +42,9 @@ typedef struct cfdart_bubble_params {
 } cfdart_bubble_params;
 int cfdart_ic_shocked_bubbles(const cfdart_desc* d, const cfdart_bubble_params* p, double* host_state);
 
+/* At most this many discs / bubbles per generator call. */
+#define CFDART_IC_MAX_DISCS 256
+
 /* Planar initial condition: cells with a*x + b*y <= c take W_under, the rest W_over
  * (allaire_2d/sourcecode/allaire_diffuse_io.cpp:332-360); primitive vectors of the model. */
 int cfdart_ic_planar(const cfdart_desc* d, const double* W_under, const double* W_over, double a,

@@ -1,0 +1,96 @@
+"""Batched execution: every grid in a mixed Euler/Allaire batch of uneven sizes gives bitwise
+the result of the same grid run alone through the plain solver (and hence the oracle), including
+grids that join late, finish early or are removed; admission respects the memory budget."""
+import math
+
+import numpy as np
+import pytest
+
+import cfdart
+import oracle
+from util import bitwise_equal, interior
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_cases(seed=0, n=8):
+    rng = np.random.default_rng(seed)
+    cases = []
+    for k in range(n):
+        nx, ny = int(rng.integers(24, 130)), int(rng.integers(24, 130))
+        if k % 4 == 3:
+            d, h = cfdart.configs.allaire_bubbles(nx, 2, seed=100 + k, ny=ny)
+            cfl = 0.5
+        else:
+            bc = tuple(int(b) for b in rng.integers(0, 2, 4))
+            d, h = cfdart.configs.euler_discs(nx, 3, (0.05, 0.2), bc, seed=100 + k, ny=ny)
+            cfl = 0.8
+        cases.append((d, h, cfl, int(rng.integers(5, 25))))
+    return cases
+
+
+def alone(d, h, cfl, steps, splitting=cfdart.ALTERNATING):
+    g = cfdart.Grid(d)
+    g.upload(h)
+    t, n = g.run(splitting, steps, cfl=cfl)
+    return g.download(), t, n
+
+
+@pytest.mark.parametrize("splitting", [cfdart.ALTERNATING, cfdart.NAIVE])
+def test_batch_equals_alone(gpu, splitting):
+    cases = mixed_cases(1)
+    b = cfdart.Batch(16)
+    slots = []
+    for d, h, cfl, steps in cases:
+        slots.append(b.add(d, h, splitting=splitting, max_steps=steps, cfl=cfl, y_strip_rows=16, x_rows_per_wave=2))
+    b.step(30)
+    b.sync()
+    for (d, h, cfl, steps), s in zip(cases, slots):
+        ref, t, n = alone(d, h, cfl, steps, splitting)
+        tb, nb, act = b.status(s)
+        print(f"case batch slot={s} model={d.model} {d.nx}x{d.ny} steps={steps}", flush=True)
+        assert nb == n == steps and tb == t and not act
+        assert bitwise_equal(interior(b.download(s)), interior(ref))
+    b.close()
+
+
+def test_batch_join_late_and_remove(gpu):
+    cases = mixed_cases(2, 6)
+    b = cfdart.Batch(8)
+    s0 = b.add(*cases[0][:2], max_steps=cases[0][3], cfl=cases[0][2])
+    s1 = b.add(*cases[1][:2], max_steps=40, cfl=cases[1][2])
+    b.step(3)
+    s2 = b.add(*cases[2][:2], max_steps=cases[2][3], cfl=cases[2][2])  # joins at global step 3
+    b.step(2)
+    b.remove(s1)  # leaves mid-run
+    s3 = b.add(*cases[3][:2], max_steps=cases[3][3], cfl=cases[3][2])  # may reuse s1's memory
+    b.step(40)
+    b.sync()
+    for s, c in ((s0, cases[0]), (s2, cases[2]), (s3, cases[3])):
+        d, h, cfl, steps = c
+        ref, t, n = alone(d, h, cfl, steps)
+        assert b.status(s)[1] == steps
+        assert bitwise_equal(interior(b.download(s)), interior(ref))
+
+
+def test_batch_matches_oracle_run_to_T(gpu):
+    d, h = cfdart.configs.euler_discs(64, 4, (0.05, 0.2), (0, 1, 0, 1), seed=5)
+    b = cfdart.Batch(2)
+    s = b.add(d, h, max_steps=100000, T=0.02, cfl=0.8)
+    b.step(200)
+    t, n, act = b.status(s)
+    ora = oracle.Sim(d, h)
+    k, _ = ora.run(cfdart.ALTERNATING, 100000, T=0.02, cfl=0.8)
+    assert not act and n == k and t == ora.t == 0.02
+    assert bitwise_equal(interior(b.download(s)), interior(ora.grid))
+
+
+def test_batch_admission_budget(gpu):
+    d, h = cfdart.configs.euler_discs(64, 2, (0.05, 0.2), (0, 0, 0, 0), seed=1)
+    one = 2 * (2 << 20)  # two state buffers of 64x64x4 doubles, rounded to 2 MiB each
+    b = cfdart.Batch(8, mem_budget_bytes=3 * one)
+    slots = [b.try_add(d, h, max_steps=5) for _ in range(4)]
+    assert slots[:3] == [0, 1, 2] and slots[3] is None
+    b.remove(1)
+    assert b.try_add(d, h, max_steps=5) == 1
+    assert b.info()[0] == 3
