@@ -24,6 +24,12 @@ for p in (os.path.join(ROOT, "cfd-art_amd"), os.path.join(ROOT, "oracle")):
         sys.path.insert(0, p)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+_T0 = time.time()
+
+
+def log(msg):
+    """Progress on stderr (the JSON result line is the only stdout output)."""
+    print(f"[bench {time.time() - _T0:7.1f}s] {msg}", file=sys.stderr, flush=True)
 
 
 def read_traffic(kernel):
@@ -58,6 +64,63 @@ def cpu_baseline(steps):
     }
 
 
+def bench_batched(args):
+    """Batched workload: all grids advance together (one persistent launch per sweep per model,
+    a captured graph per step) versus the same grids one at a time with the plain solver."""
+    import numpy as np
+    import cfdart
+
+    cases = cfdart.configs.batched_mix(args.grids)
+    table = cfdart.autotune.tune(classes=(128, 256, 512, 1024), log=log)
+    steps = args.batch_steps
+    cells = [c[0].nx * c[0].ny for c in cases]
+    work = float(sum(cells)) * steps
+    log(f"batched: {len(cases)} grids, {sum(cells)/1e6:.1f} M cells, {steps} steps each, "
+        f"{sum(1 for c in cases if c[0].model == 0)} Euler")
+
+    b = cfdart.Batch(len(cases))
+    slots = []
+    for d, h, cfl in cases:
+        yh, xr = cfdart.autotune.shape_for(table, d.model, d.nx, d.ny)
+        slots.append(b.add(d, h, splitting=cfdart.ALTERNATING, max_steps=steps, cfl=cfl, y_strip_rows=yh,
+                           x_rows_per_wave=xr))
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.sync()
+    t_batch = time.perf_counter() - t0
+    log(f"batched: {t_batch:.3f} s -> {work / t_batch / 1e9:.2f} G cell-updates/s")
+
+    t_plain = 0.0
+    same = 0
+    for (d, h, cfl), s in zip(cases, slots):
+        g = cfdart.Grid(d)
+        yh, xr = cfdart.autotune.shape_for(table, d.model, d.nx, d.ny)
+        g.set_tile(yh, xr)
+        g.upload(h)
+        g.sync()
+        t0 = time.perf_counter()
+        g.run(cfdart.ALTERNATING, steps, cfl=cfl)
+        t_plain += time.perf_counter() - t0
+        a = g.download()[2:-2, 2:-2]
+        bb = b.download(s)[2:-2, 2:-2]
+        same += int(np.array_equal(a.view(np.uint64), bb.view(np.uint64)))
+        g.close()
+    log(f"one at a time: {t_plain:.3f} s -> {work / t_plain / 1e9:.2f} G cell-updates/s; bitwise equal {same}/{len(cases)}")
+    return {
+        "metric": "aggregate cell-updates/s, batched vs one-at-a-time (FP64)",
+        "value": work / t_batch, "unit": "cell-updates/s", "n_gpus": 1, "steps": steps,
+        "higher_is_better": True, "dtype": "f64", "data": "synthetic (seeded disc/bubble generators)",
+        "config": {"workload": "batched", "grids": len(cases), "cells": sum(cells),
+                   "euler": sum(1 for c in cases if c[0].model == 0), "sides": "log-uniform [128,1024]",
+                   "splitting": "alternating"},
+        "one_at_a_time": {"value": work / t_plain, "seconds": t_plain},
+        "speedup_vs_one_at_a_time": t_plain / t_batch,
+        "bitwise_equal_alone": f"{same}/{len(cases)}",
+        "autotune": {str(m): {str(c): v for c, v in t.items()} for m, t in table.items()},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,6 +128,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-steps", type=int, default=12, help="oracle steps for the CPU baseline (0: skip)")
     ap.add_argument("--strip-rows", type=int, default=0)
+    ap.add_argument("--workload", default="euler_2048", choices=["euler_2048", "batched"])
+    ap.add_argument("--grids", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain device copy of the state size (PMC calibration)")
     args = ap.parse_args()
@@ -88,12 +154,18 @@ def main():
 
     if cfdart.device_count() < 1:
         raise SystemExit("bench.py: no GPU visible")
+    if args.workload == "batched":
+        print(json.dumps(bench_batched(args)), flush=True)
+        return
     d, h = cfdart.configs.euler_2048()
     cells = d.nx * d.ny
+    tuned = cfdart.autotune.tune_one(cfdart.EULER, 2048, log=log, case=(d, h))
     g = cfdart.Grid(d)
+    g.set_tile(tuned["y_strip_rows"], tuned["x_rows_per_wave"])
     if args.strip_rows:
         g.set_strip_rows(args.strip_rows)
     g.upload(h)
+    log(f"euler_2048: warm-up {args.warmup} steps, timed {args.steps} steps")
     g.run(cfdart.ALTERNATING, args.warmup, cfl=0.8)
     g.set_profiling(True)
     g.sync()
@@ -139,6 +211,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "kernels": per_kernel,
+        "tile": {"y_strip_rows": tuned["y_strip_rows"], "x_rows_per_wave": tuned["x_rows_per_wave"]},
         "fallback_counters": counters,
     }
     if args.cpu_steps > 0:

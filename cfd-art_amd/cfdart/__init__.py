@@ -10,4 +10,4 @@ from ._lib import (  # noqa: F401
     NAIVE, STRANG, ALTERNATING, ncomp, make_desc, random_discs, shocked_bubbles, planar,
     prim_to_cons, device_count,
 )
-from . import configs  # noqa: F401
+from . import autotune, configs  # noqa: F401

@@ -98,6 +98,7 @@ def lib():
         L.cfdart_batch_status.argtypes = [C.c_void_p, C.c_int, dp, P(C.c_int), P(C.c_int)]
         L.cfdart_batch_stats.argtypes = [C.c_void_p, C.c_int, P(C.c_longlong)]
         L.cfdart_batch_sync.argtypes = [C.c_void_p]
+        L.cfdart_batch_set_tile.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
         L.cfdart_batch_info.argtypes = [C.c_void_p, P(C.c_int), P(C.c_size_t), P(C.c_int)]
         L.cfdart_batch_frame.argtypes = [C.c_void_p, C.c_int, dp]
         L.cfdart_batch_frames_done.argtypes = [C.c_void_p]
@@ -107,7 +108,7 @@ def lib():
         L.cfdart_host_alloc.restype = C.c_void_p
         L.cfdart_host_free.argtypes = [C.c_void_p]
         L.cfdart_host_free.restype = None
-        L.cfdart_time_tile.argtypes = [P(Desc), C.c_int, C.c_int, C.c_int, dp, dp]
+        L.cfdart_time_tile.argtypes = [P(Desc), dp, C.c_int, C.c_int, C.c_int, dp, dp]
         _lib = L
     return _lib
 
@@ -322,6 +323,9 @@ class Batch:
         _check(lib().cfdart_batch_stats(self.h, slot, c))
         return list(c)
 
+    def set_tile(self, slot, y_strip_rows, x_rows_per_wave):
+        _check(lib().cfdart_batch_set_tile(self.h, slot, y_strip_rows, x_rows_per_wave))
+
     def info(self):
         live = C.c_int()
         b = C.c_size_t()
@@ -336,8 +340,10 @@ class Batch:
         _check(lib().cfdart_batch_frames_done(self.h))
 
 
-def time_tile(desc, y_strip_rows, x_rows_per_wave, reps=5):
+def time_tile(desc, y_strip_rows, x_rows_per_wave, reps=5, state=None):
     ux = C.c_double()
     uy = C.c_double()
-    _check(lib().cfdart_time_tile(C.byref(desc), y_strip_rows, x_rows_per_wave, reps, C.byref(ux), C.byref(uy)))
+    sp = None if state is None else _dptr(np.ascontiguousarray(state, dtype=np.float64))
+    _check(lib().cfdart_time_tile(C.byref(desc), sp, y_strip_rows, x_rows_per_wave, reps, C.byref(ux),
+                                  C.byref(uy)))
     return ux.value, uy.value

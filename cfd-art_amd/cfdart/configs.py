@@ -56,3 +56,23 @@ def allaire_bubbles(n=1024, n_bubbles=16, seed=1, ny=None, wave_speeds=L.PVRS, l
 
 def allaire_1024(seed=1):
     return allaire_bubbles(1024, 16, seed)
+
+
+def batched_mix(n_grids=256, seed=2024, lo=128, hi=1024, euler_frac=0.75):
+    """The batched workload: n_grids independent grids, each Euler (probability euler_frac) or
+    Allaire, with nx and ny drawn independently log-uniform in [lo, hi] (square cells, domain
+    [0,1] x [0,ny/nx]); Euler grids carry 8 seeded discs (radii U[0.03,0.10]), Allaire grids 16
+    seeded air bubbles behind a planar shock. Returns [(desc, state, cfl)]."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_grids):
+        nx = int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+        ny = int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+        if rng.uniform() < euler_frac:
+            d, h = euler_discs(nx, 8, (0.03, 0.10), (0, 0, 0, 0), seed=seed * 1000 + k, ny=ny)
+            out.append((d, h, 0.8))
+        else:
+            d, h = allaire_bubbles(nx, 16, seed=seed * 1000 + k, ny=ny)
+            out.append((d, h, 0.5))
+    return out

@@ -77,18 +77,32 @@ template <int M, bool LAST>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void batch_sweep_kernel(
     BatchGrid* grids, const Tile* tiles, BatchQueue* q, int phase)
 {
-    __shared__ int s_t;
+    __shared__ int s_t, s_axis;
     for (;;) {
         __syncthreads();
-        if (threadIdx.x == 0) s_t = (int)atomicAdd(&q->head[M][phase], 1u);
+        if (threadIdx.x == 0) {
+            // one thread claims tiles until it finds work: tiles of finished grids, and tiles of
+            // the other axis (a grid's axis per phase follows its own step parity), are skipped
+            // without synchronising the workgroup
+            int t, axis = 0;
+            for (;;) {
+                t = (int)atomicAdd(&q->head[M][phase], 1u);
+                if (t >= q->ntiles[M]) { t = -1; break; }
+                const Tile tc = tiles[t];
+                const BatchGrid* gc = grids + tc.grid;
+                if (!gc->live || !gc->ctrl.active) continue;
+                axis = (gc->splitting == 2 && (gc->ctrl.n & 1)) ? 1 - phase : phase;
+                if (tc.axis == axis) break;
+            }
+            s_t = t;
+            s_axis = axis;
+        }
         __syncthreads();
         const int t = s_t;
-        if (t >= q->ntiles[M]) break;
+        if (t < 0) break;
         const Tile tl = tiles[t];
         BatchGrid* bg = grids + tl.grid;
-        if (!bg->live || !bg->ctrl.active) continue;
-        const int axis = (bg->splitting == 2 && (bg->ctrl.n & 1)) ? 1 - phase : phase;
-        if (tl.axis != axis) continue;
+        const int axis = s_axis;
         const GridView g = batch_view(bg, phase);
         const double dt = bg->ctrl.dt;
         Tally ta;
@@ -100,7 +114,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void b
             const int y0 = tl.b * bg->yH, y1 = min(y0 + bg->yH, bg->ny);
             y_tile<M, LAST>(g, dt, tl.a, y0, y1, ta);
         }
-        ta.flush(&bg->ctrl, LAST);
+        ta.flush(&bg->ctrl, LAST, t);
     }
 }
 
@@ -126,9 +140,9 @@ __global__ void batch_finalize_kernel(BatchGrid* grids, int cap, BatchQueue* q)
     c->n = n;
     bg->steps_left -= 1;
     c->active = (t < c->T) && bg->steps_left > 0;
-    if (c->active) c->dt = dt_from_maxima(c, n, t, bg->dx, bg->dy);
-    c->maxu = 0;
-    c->maxv = 0;
+    double mu, mv;
+    fold_maxima(c, mu, mv);
+    if (c->active) c->dt = dt_from_maxima(c, mu, mv, n, t, bg->dx, bg->dy);
 }
 
 __global__ void batch_reset_queue_kernel(BatchQueue* q, int ntiles_euler, int ntiles_allaire)
@@ -145,10 +159,10 @@ __global__ void batch_admit_kernel(BatchGrid* bg)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     Ctrl* c = &bg->ctrl;
+    double mu, mv;
+    fold_maxima(c, mu, mv);
     c->active = (c->t < c->T) && bg->steps_left > 0;
-    if (c->active) c->dt = dt_from_maxima(c, c->n, c->t, bg->dx, bg->dy);
-    c->maxu = 0;
-    c->maxv = 0;
+    if (c->active) c->dt = dt_from_maxima(c, mu, mv, c->n, c->t, bg->dx, bg->dy);
 }
 
 // Frame snapshot: contiguous copy of the interior of buf[0] (SoA planes, pitch removed).
@@ -593,6 +607,20 @@ int cfdart_batch_download(cfdart_batch* b, int slot, double* host)
     return 0;
 }
 
+int cfdart_batch_set_tile(cfdart_batch* b, int slot, int yH, int xrows)
+{
+    int rc = check_slot(b, slot);
+    if (rc) return rc;
+    if (yH < 1 || yH > 4096 || xrows < 1 || xrows > 256) return bfail(CFDART_E_ARG, "tile shape out of range");
+    BatchGrid& g = b->h_grids[slot];
+    g.yH = yH;
+    g.xrows = xrows;
+    // only the two tile-shape fields change on the device
+    BCHK(hipMemcpyAsync(&(b->d_grids + slot)->yH, &g.yH, 2 * sizeof(int), hipMemcpyHostToDevice, b->st));
+    BCHK(hipStreamSynchronize(b->st));
+    return rebuild_tiles(b);
+}
+
 int cfdart_batch_info(cfdart_batch* b, int* live, size_t* bytes, int* tiles)
 {
     if (!b) return bfail(CFDART_E_ARG, "null batch");
@@ -652,7 +680,8 @@ void cfdart_host_free(void* p)
     if (p) (void)hipHostFree(p);
 }
 
-int cfdart_time_tile(const cfdart_desc* d, int yH, int xrows, int reps, double* us_x, double* us_y)
+int cfdart_time_tile(const cfdart_desc* d, const double* host_state, int yH, int xrows, int reps, double* us_x,
+                     double* us_y)
 {
     if (!d || !us_x || !us_y || reps < 1) return bfail(CFDART_E_ARG, "bad argument");
     cfdart_grid* g = nullptr;
@@ -672,7 +701,7 @@ int cfdart_time_tile(const cfdart_desc* d, int yH, int xrows, int reps, double* 
     cfdart_prim_to_cons(d, W, U);
     for (size_t c = 0; c < h.size() / nc; ++c)
         for (int k = 0; k < nc; ++k) h[c * nc + k] = U[k];
-    rc = cfdart_upload(g, h.data());
+    rc = cfdart_upload(g, host_state ? host_state : h.data());
     if (!rc) rc = cfdart_set_tile(g, yH, xrows);
     if (!rc) rc = cfdart_set_profiling(g, 1);
     double dt = 0.0;

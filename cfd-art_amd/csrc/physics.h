@@ -509,32 +509,25 @@ CFD_HD void hllc(const Material& m, int wave_speeds, const Side<M>& L, const Sid
 // ---------------------------------------------------------------- face stage
 
 // Face between cell a (left) and cell b (right): M3 combination, M5 fallback, HLLC.
-// flags: bit0 slope fallback taken, bit1 evolve fallback taken.
+// flags: bit0 slope fallback taken, bit1 evolve fallback taken. The two fallbacks only choose
+// the Riemann-problem inputs; there is one HLLC evaluation per face.
 template <int M>
 CFD_HD int face_stage(const Material& m, const SchemeOpts& o, double s,
                       const Prim<M>& Pa, const double* Ua, const CellStage<M>& a,
                       const Prim<M>& Pb, const double* Ub, const CellStage<M>& b, FaceOut<M>& out)
 {
     int flags = 0;
-    if (a.fR && b.fL) {
-        if (a.R.ok && b.L.ok) {
-            hllc<M>(m, o.wave_speeds, a.R, b.L, out);
-            return 0;
-        }
-    } else {
+    Side<M> L = a.R, R = b.L;
+    if (!(a.fR && b.fL)) {
         flags |= 1;
-        Side<M> L, R;
         unsloped_side<M>(m, s, Pa, true, L);
         unsloped_side<M>(m, s, Pb, false, R);
-        if (L.ok && R.ok) {
-            hllc<M>(m, o.wave_speeds, L, R, out);
-            return flags;
-        }
     }
-    flags |= 2;
-    Side<M> L, R;
-    side_from_raw<M>(m, Ua, L);
-    side_from_raw<M>(m, Ub, R);
+    if (!(L.ok && R.ok)) {
+        flags |= 2;
+        side_from_raw<M>(m, Ua, L);
+        side_from_raw<M>(m, Ub, R);
+    }
     hllc<M>(m, o.wave_speeds, L, R, out);
     return flags;
 }

@@ -62,23 +62,37 @@ __device__ __forceinline__ int wave_sum(int v)
 struct Tally {
     double su = 0.0, sv = 0.0;  // running max of |u|+c, |v|+c (NaN never enters: smax keeps the old value)
     int slope = 0, evolve = 0, repair = 0, bad = 0;
-    __device__ void flush(Ctrl* c, bool last)
+    // Workgroup-wide reduction, then one atomic per workgroup into a slot of the control block.
+    // Every thread of the workgroup must call it (it synchronises the workgroup).
+    __device__ void flush(Ctrl* c, bool last, int slot)
     {
-        int lane = threadIdx.x & 63;
-        if (last) {
-            double mu = wave_max(su), mv = wave_max(sv);
-            if (lane == 0) {
-                atomicMax(&c->maxu, (unsigned long long)__double_as_longlong(mu));
-                atomicMax(&c->maxv, (unsigned long long)__double_as_longlong(mv));
-            }
-        }
+        __shared__ double s_m[2][16];
+        __shared__ int s_c[4][16];
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+        double mu = 0.0, mv = 0.0;
+        if (last) { mu = wave_max(su); mv = wave_max(sv); }
         int a = wave_sum(slope), b = wave_sum(evolve), d = wave_sum(repair), e = wave_sum(bad);
         if (lane == 0) {
-            if (a) atomicAdd(&c->cnt[0], (unsigned long long)a);
-            if (b) atomicAdd(&c->cnt[1], (unsigned long long)b);
-            if (d) atomicAdd(&c->cnt[2], (unsigned long long)d);
-            if (e) { atomicAdd(&c->cnt[3], (unsigned long long)e); atomicOr(&c->error, 1); }
+            s_m[0][w] = mu; s_m[1][w] = mv;
+            s_c[0][w] = a; s_c[1][w] = b; s_c[2][w] = d; s_c[3][w] = e;
         }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int i = 1; i < nw; ++i) {
+                s_m[0][0] = smax(s_m[0][0], s_m[0][i]);
+                s_m[1][0] = smax(s_m[1][0], s_m[1][i]);
+                for (int k = 0; k < 4; ++k) s_c[k][0] += s_c[k][i];
+            }
+            const int sl = slot & (MAX_SLOTS - 1);
+            if (last) {
+                atomicMax(&c->maxu[sl], (unsigned long long)__double_as_longlong(s_m[0][0]));
+                atomicMax(&c->maxv[sl], (unsigned long long)__double_as_longlong(s_m[1][0]));
+            }
+            for (int k = 0; k < 4; ++k)
+                if (s_c[k][0]) atomicAdd(&c->cnt[k], (unsigned long long)s_c[k][0]);
+            if (s_c[3][0]) atomicOr(&c->error, 1);
+        }
+        __syncthreads();
     }
 };
 
@@ -129,11 +143,22 @@ __device__ __forceinline__ void x_tile(const GridView& g, double dt, int tx, int
     const bool face_counted = (lane >= 2 && lane < 62 && x <= g.nx) || (lane == 62 && x == g.nx);
     const int yend = min(row0 + nrows, g.ny);
 
+    // rows are independent: the next row's state is loaded while this one is computed
+    double Unext[NC];
+    if (row0 < yend) {
+        const double* base = g.cur + (long long)row0 * g.pitch + xs;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) Unext[k] = base[k * g.plane];
+    }
     for (int y = row0; y < yend; ++y) {
         double U[NC];
-        const double* base = g.cur + (long long)y * g.pitch + xs;
 #pragma unroll
-        for (int k = 0; k < NC; ++k) U[k] = base[k * g.plane];
+        for (int k = 0; k < NC; ++k) U[k] = Unext[k];
+        if (y + 1 < yend) {
+            const double* base = g.cur + (long long)(y + 1) * g.pitch + xs;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) Unext[k] = base[k * g.plane];
+        }
         if (neg) U[MN] = -U[MN];
 
         Prim<M> P;
@@ -306,10 +331,24 @@ __device__ __forceinline__ void y_tile(const GridView& g, double dt, int bx, int
 }
 
 // dt from the reduced maxima (E/sourcecode/euler_timestep.cpp:35-43).
-__device__ __forceinline__ double dt_from_maxima(const Ctrl* c, int n, double t, double dx, double dy)
+// Folds the slot maxima of the control block (and clears them for the next reduction).
+__device__ __forceinline__ void fold_maxima(Ctrl* c, double& mu, double& mv)
 {
-    double mu = __longlong_as_double((long long)c->maxu);
-    double mv = __longlong_as_double((long long)c->maxv);
+    unsigned long long a = 0, b = 0;
+    for (int i = 0; i < MAX_SLOTS; ++i) {
+        a = c->maxu[i] > a ? c->maxu[i] : a;
+        b = c->maxv[i] > b ? c->maxv[i] : b;
+        c->maxu[i] = 0;
+        c->maxv[i] = 0;
+    }
+    mu = __longlong_as_double((long long)a);
+    mv = __longlong_as_double((long long)b);
+}
+
+// dt from the folded maxima (E/sourcecode/euler_timestep.cpp:35-43).
+__device__ __forceinline__ double dt_from_maxima(const Ctrl* c, double mu, double mv, int n, double t, double dx,
+                                                 double dy)
+{
     double CFL = c->cfl;
     if (n < 5) CFL = smin(CFL, 0.2);
     double dt = CFL * smin(dx / mu, dy / mv);

@@ -10,10 +10,12 @@ namespace cfd {
 // Per-grid device control block: the state of the reference's time loop
 // (E/headers/simulation.hpp:48-61) kept on the device so a run of many steps needs no
 // host round trip per step.
+constexpr int MAX_SLOTS = 64;  // spread of the wave-speed maxima over addresses (no hot atomic)
 struct Ctrl {
     double t, dt, T, cfl;
     int n, active;
-    unsigned long long maxu, maxv;  // bit patterns of non-negative doubles (max-reduced)
+    // bit patterns of non-negative doubles, max-reduced by workgroups into slot (id % MAX_SLOTS)
+    unsigned long long maxu[MAX_SLOTS], maxv[MAX_SLOTS];
     unsigned long long cnt[4];      // slope fallbacks, evolve fallbacks, repairs, unphysical
     double mass_partial_unused;
     int error;

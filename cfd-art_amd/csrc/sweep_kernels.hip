@@ -30,7 +30,7 @@ __global__ __launch_bounds__(256) void sweep_x_kernel(GridView g, double dt_arg,
     }
     Tally tl;
     x_tile<M, LAST>(g, dt, blockIdx.x, row0, xrows, tl);
-    tl.flush(g.ctrl, LAST);
+    tl.flush(g.ctrl, LAST, blockIdx.x + blockIdx.y * gridDim.x);
 }
 
 template <int M, bool LAST>
@@ -47,7 +47,7 @@ __global__ __launch_bounds__(YBLOCK) void sweep_y_kernel(GridView g, double dt_a
     }
     Tally tl;
     y_tile<M, LAST>(g, dt, blockIdx.x, y0, y1, tl);
-    tl.flush(g.ctrl, LAST);
+    tl.flush(g.ctrl, LAST, blockIdx.x + blockIdx.y * gridDim.x);
 }
 
 // ------------------------------------------------------------------ dt reduction / time loop
@@ -75,17 +75,17 @@ __global__ __launch_bounds__(256) void reduce_dt_kernel(GridView g, double* part
         m1 += U[0];
         if (M == ALLAIRE) m2 += U[1];
     }
-    double mu = wave_max(su), mv = wave_max(sv);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) { m1 += __shfl_xor(m1, o); m2 += __shfl_xor(m2, o); }
     int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane == 0) {
-        atomicMax(&g.ctrl->maxu, (unsigned long long)__double_as_longlong(mu));
-        atomicMax(&g.ctrl->maxv, (unsigned long long)__double_as_longlong(mv));
         sm[0][w] = m1;
         sm[1][w] = m2;
     }
-    __syncthreads();
+    Tally tl;
+    tl.su = su;
+    tl.sv = sv;
+    tl.flush(g.ctrl, true, blockIdx.x);
     if (threadIdx.x == 0) {
         partials[2 * blockIdx.x] = ((sm[0][0] + sm[0][1]) + (sm[0][2] + sm[0][3]));
         partials[2 * blockIdx.x + 1] = ((sm[1][0] + sm[1][1]) + (sm[1][2] + sm[1][3]));
@@ -96,10 +96,10 @@ __global__ __launch_bounds__(256) void reduce_dt_kernel(GridView g, double* part
 __global__ void init_dt_kernel(Ctrl* c, double dx, double dy)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double mu, mv;
+    fold_maxima(c, mu, mv);
     c->active = c->t < c->T;
-    if (c->active) c->dt = dt_from_maxima(c, c->n, c->t, dx, dy);
-    c->maxu = 0;
-    c->maxv = 0;
+    if (c->active) c->dt = dt_from_maxima(c, mu, mv, c->n, c->t, dx, dy);
 }
 
 // End of a step (E/headers/simulation.hpp:68-70): t += dt, n++, snap to T, loop test, and the
@@ -114,9 +114,9 @@ __global__ void finalize_step_kernel(Ctrl* c, double dx, double dy)
     c->t = t;
     c->n = n;
     c->active = t < c->T;
-    if (c->active) c->dt = dt_from_maxima(c, n, t, dx, dy);
-    c->maxu = 0;
-    c->maxv = 0;
+    double mu, mv;
+    fold_maxima(c, mu, mv);
+    if (c->active) c->dt = dt_from_maxima(c, mu, mv, n, t, dx, dy);
 }
 
 __global__ void set_run_kernel(Ctrl* c, double T, double cfl)

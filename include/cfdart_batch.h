@@ -50,6 +50,8 @@ int cfdart_batch_status(cfdart_batch* b, int slot, double* t, int* n, int* activ
 /* Fallback counters of a grid (see CFDART_CNT_*), 4 values; synchronous. */
 int cfdart_batch_stats(cfdart_batch* b, int slot, long long* counters);
 int cfdart_batch_sync(cfdart_batch* b);
+/* Changes a grid's tile shape (used by the autotuner between steps). */
+int cfdart_batch_set_tile(cfdart_batch* b, int slot, int y_strip_rows, int x_rows_per_wave);
 /* Live grids, device bytes held by grid state, tiles in the work-list. */
 int cfdart_batch_info(cfdart_batch* b, int* live, size_t* bytes, int* tiles);
 /* Frame snapshot: device-to-device copy of a grid's state into a staging buffer on the batch
@@ -65,10 +67,10 @@ void* cfdart_host_alloc(size_t bytes);
 void cfdart_host_free(void* p);
 
 /* Autotuner measurement: times one sweep of each axis for a grid of this description with a
- * given tile shape (synthetic uniform state), returning the mean kernel time in microseconds
- * over 'reps' launches per axis. */
-int cfdart_time_tile(const cfdart_desc* desc, int y_strip_rows, int x_rows_per_wave, int reps,
-                     double* us_x, double* us_y);
+ * given tile shape, on host_state (format of cfdart.h; NULL: a uniform state), returning the
+ * mean kernel time in microseconds over 'reps' launches per axis. */
+int cfdart_time_tile(const cfdart_desc* desc, const double* host_state, int y_strip_rows,
+                     int x_rows_per_wave, int reps, double* us_x, double* us_y);
 
 #ifdef __cplusplus
 }
