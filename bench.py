@@ -78,10 +78,23 @@ def bench_batched(args):
     log(f"batched: {len(cases)} grids, {sum(cells)/1e6:.1f} M cells, {steps} steps each, "
         f"{sum(1 for c in cases if c[0].model == 0)} Euler")
 
+    # Tile shapes for the batch: tuned per model on a batch of that model's grids alone (the
+    # per-grid table above is tuned for a grid that has the GPU to itself; in a batch the
+    # per-tile cost of the work-list favours larger tiles).
+    batch_shape = {}
+    for model in (cfdart.EULER, cfdart.ALLAIRE):
+        sub = [c for c in cases if c[0].model == model]
+        if not sub:
+            continue
+        tb = cfdart.Batch(len(sub))
+        ts = [tb.add(d, h, splitting=cfdart.ALTERNATING, max_steps=steps, cfl=cfl) for d, h, cfl in sub]
+        batch_shape.update(cfdart.autotune.tune_batch(tb, {model: ts}, steps=3, log=log))
+        tb.close()
+
     b = cfdart.Batch(len(cases))
     slots = []
     for d, h, cfl in cases:
-        yh, xr = cfdart.autotune.shape_for(table, d.model, d.nx, d.ny)
+        yh, xr = batch_shape[d.model]
         slots.append(b.add(d, h, splitting=cfdart.ALTERNATING, max_steps=steps, cfl=cfl, y_strip_rows=yh,
                            x_rows_per_wave=xr))
     b.sync()
@@ -118,6 +131,7 @@ def bench_batched(args):
         "speedup_vs_one_at_a_time": t_plain / t_batch,
         "bitwise_equal_alone": f"{same}/{len(cases)}",
         "autotune": {str(m): {str(c): v for c, v in t.items()} for m, t in table.items()},
+        "batch_tile": {("euler" if m == 0 else "allaire"): list(v) for m, v in batch_shape.items()},
     }
 
 
@@ -128,6 +142,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-steps", type=int, default=12, help="oracle steps for the CPU baseline (0: skip)")
     ap.add_argument("--strip-rows", type=int, default=0)
+    ap.add_argument("--tile", default="", help="Y_STRIP_ROWS,X_ROWS_PER_WAVE: fixed tile shape, no autotuning")
     ap.add_argument("--workload", default="euler_2048", choices=["euler_2048", "batched"])
     ap.add_argument("--grids", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=500)
@@ -159,7 +174,11 @@ def main():
         return
     d, h = cfdart.configs.euler_2048()
     cells = d.nx * d.ny
-    tuned = cfdart.autotune.tune_one(cfdart.EULER, 2048, log=log, case=(d, h))
+    if args.tile:
+        yh, xr = (int(v) for v in args.tile.split(","))
+        tuned = {"y_strip_rows": yh, "x_rows_per_wave": xr}
+    else:
+        tuned = cfdart.autotune.tune_one(cfdart.EULER, 2048, log=log, case=(d, h))
     g = cfdart.Grid(d)
     g.set_tile(tuned["y_strip_rows"], tuned["x_rows_per_wave"])
     if args.strip_rows:
