@@ -40,6 +40,18 @@ template <int M> __device__ __forceinline__ void side_from_left(const Side<M>& s
 
 // Ghost index of the reference's pre_sweep: interior index i of an axis of n cells, with
 // i in [-2, n+2) mapped to its mirror. Indices further out (dead lanes) are clamped in range.
+// Workgroups go to the 8 XCDs (each with its own L2) round-robin in order of linear id.
+// xcd_order maps the linear id b of n workgroups to a tile order in which each XCD receives
+// one contiguous run of tiles (the last n % 8 workgroups keep their id), so tiles that
+// share halo cache lines are fetched into the same L2.
+__device__ __forceinline__ int xcd_order(int b, int n)
+{
+    constexpr int NXCD = 8;
+    const int q = n / NXCD;
+    if (b >= q * NXCD) return b;
+    return (b % NXCD) * q + b / NXCD;
+}
+
 __device__ __forceinline__ int mirror(int i, int n)
 {
     int r = i < 0 ? -1 - i : (i >= n ? 2 * n - 1 - i : i);

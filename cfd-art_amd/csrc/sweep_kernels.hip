@@ -20,16 +20,20 @@ template <int M, bool LAST>
 __global__ __launch_bounds__(256) void sweep_x_kernel(GridView g, double dt_arg, double dt_factor, int use_ctrl,
                                                      int xrows)
 {
+    // tiles in row-major order: neighbours along x (which share halo columns) are consecutive
+    const int b = blockIdx.x + blockIdx.y * gridDim.x;
+    const int L = xcd_order(b, gridDim.x * gridDim.y);
+    const int tx = L % gridDim.x, ty = L / gridDim.x;
     const int wave = threadIdx.x >> 6;
-    const int row0 = (blockIdx.y * 4 + wave) * xrows;
+    const int row0 = (ty * 4 + wave) * xrows;
     double dt;
     if (!sweep_dt(g, dt_arg, dt_factor, use_ctrl, dt)) {
-        const int lane = threadIdx.x & 63, x = blockIdx.x * XTILE - 2 + lane;
+        const int lane = threadIdx.x & 63, x = tx * XTILE - 2 + lane;
         copy_rows<M>(g, x, lane >= 2 && lane < 62 && x < g.nx, row0, min(row0 + xrows, g.ny));
         return;
     }
     Tally tl;
-    x_tile<M, LAST>(g, dt, blockIdx.x, row0, xrows, tl);
+    x_tile<M, LAST>(g, dt, tx, row0, xrows, tl);
     tl.flush(g.ctrl, LAST, blockIdx.x + blockIdx.y * gridDim.x);
 }
 
@@ -37,16 +41,20 @@ template <int M, bool LAST>
 __global__ __launch_bounds__(YBLOCK) void sweep_y_kernel(GridView g, double dt_arg, double dt_factor,
                                                         int use_ctrl, int H)
 {
-    const int y0 = blockIdx.y * H;
+    // tiles in column-major order: neighbours along y (which share halo rows) are consecutive
+    const int b = blockIdx.x + blockIdx.y * gridDim.x;
+    const int L = xcd_order(b, gridDim.x * gridDim.y);
+    const int bx = L / gridDim.y, by = L % gridDim.y;
+    const int y0 = by * H;
     const int y1 = min(y0 + H, g.ny);
     double dt;
     if (!sweep_dt(g, dt_arg, dt_factor, use_ctrl, dt)) {
-        const int x = blockIdx.x * YBLOCK + threadIdx.x;
+        const int x = bx * YBLOCK + threadIdx.x;
         copy_rows<M>(g, x, x < g.nx, y0, y1);
         return;
     }
     Tally tl;
-    y_tile<M, LAST>(g, dt, blockIdx.x, y0, y1, tl);
+    y_tile<M, LAST>(g, dt, bx, y0, y1, tl);
     tl.flush(g.ctrl, LAST, blockIdx.x + blockIdx.y * gridDim.x);
 }
 
