@@ -86,12 +86,20 @@ def lib():
         L.cfdart_ic_planar.argtypes = [P(Desc), dp, dp, C.c_double, C.c_double, C.c_double, dp]
         L.cfdart_prim_to_cons.argtypes = [P(Desc), dp, dp]
         L.cfdart_set_tile.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.cfdart_init_random_discs.argtypes = [C.c_void_p, P(DiscParams)]
+        L.cfdart_init_shocked_bubbles.argtypes = [C.c_void_p, P(BubbleParams)]
+        L.cfdart_checksum.argtypes = [C.c_void_p, P(C.c_ulonglong)]
+        L.cfdart_frame.argtypes = [C.c_void_p, C.c_void_p]
         # batch (include/cfdart_batch.h)
         L.cfdart_batch_create.argtypes = [C.c_int, C.c_size_t, P(C.c_void_p)]
         L.cfdart_batch_destroy.argtypes = [C.c_void_p]
         L.cfdart_batch_destroy.restype = None
         L.cfdart_batch_add.argtypes = [C.c_void_p, P(Desc), dp, C.c_int, C.c_int, C.c_double, C.c_double,
                                        C.c_int, C.c_int, P(C.c_int)]
+        L.cfdart_batch_add_generated.argtypes = [C.c_void_p, P(Desc), P(DiscParams), P(BubbleParams), C.c_int,
+                                                 C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, P(C.c_int)]
+        L.cfdart_batch_poll.argtypes = [C.c_void_p, P(C.c_int), P(C.c_int)]
+        L.cfdart_batch_checksum.argtypes = [C.c_void_p, C.c_int, P(C.c_ulonglong)]
         L.cfdart_batch_remove.argtypes = [C.c_void_p, C.c_int]
         L.cfdart_batch_step.argtypes = [C.c_void_p, C.c_int]
         L.cfdart_batch_download.argtypes = [C.c_void_p, C.c_int, dp]
@@ -231,6 +239,18 @@ class Grid:
         _check(lib().cfdart_run(self.h, splitting, max_steps, T, cfl, C.byref(t), C.byref(n)))
         return t.value, n.value
 
+    def init_generated(self, params):
+        """Device-side initial state from DiscParams (Euler) or BubbleParams (Allaire)."""
+        if isinstance(params, DiscParams):
+            _check(lib().cfdart_init_random_discs(self.h, C.byref(params)))
+        else:
+            _check(lib().cfdart_init_shocked_bubbles(self.h, C.byref(params)))
+
+    def checksum(self):
+        v = C.c_ulonglong()
+        _check(lib().cfdart_checksum(self.h, C.byref(v)))
+        return v.value
+
     def stats(self):
         c = (C.c_longlong * 4)()
         _check(lib().cfdart_stats(self.h, c))
@@ -287,6 +307,29 @@ class Batch:
                                       y_strip_rows, x_rows_per_wave, C.byref(slot)))
         self.descs[slot.value] = desc
         return slot.value
+
+    def add_generated(self, desc, params, splitting=ALTERNATING, max_steps=1, T=math.inf, cfl=0.8,
+                      y_strip_rows=0, x_rows_per_wave=0):
+        """Admits a grid whose initial state the device generates from DiscParams (Euler) or
+        BubbleParams (Allaire). Returns the slot id; CfdartError (code -2) when refused."""
+        slot = C.c_int()
+        dp = C.byref(params) if isinstance(params, DiscParams) else None
+        bp = C.byref(params) if isinstance(params, BubbleParams) else None
+        _check(lib().cfdart_batch_add_generated(self.h, C.byref(desc), dp, bp, splitting, max_steps, T, cfl,
+                                                y_strip_rows, x_rows_per_wave, C.byref(slot)))
+        self.descs[slot.value] = desc
+        return slot.value
+
+    def poll(self, capacity):
+        steps = (C.c_int * capacity)()
+        flags = (C.c_int * capacity)()
+        _check(lib().cfdart_batch_poll(self.h, steps, flags))
+        return list(steps), list(flags)
+
+    def checksum(self, slot):
+        v = C.c_ulonglong()
+        _check(lib().cfdart_batch_checksum(self.h, slot, C.byref(v)))
+        return v.value
 
     def try_add(self, *a, **k):
         try:
@@ -347,3 +390,17 @@ def time_tile(desc, y_strip_rows, x_rows_per_wave, reps=5, state=None):
     _check(lib().cfdart_time_tile(C.byref(desc), sp, y_strip_rows, x_rows_per_wave, reps, C.byref(ux),
                                   C.byref(uy)))
     return ux.value, uy.value
+
+
+def state_checksum(state, ng=2):
+    """Host restatement of the device state checksum (cfdart_checksum) for a host state in the
+    cfdart.h layout: sum mod 2^64 of mix64(bits ^ (index * 0x9e3779b97f4a7c15)) over the
+    interior values in [component][y][x] order, mix64 being the splitmix64 finaliser."""
+    a = np.ascontiguousarray(np.moveaxis(state[ng:-ng, ng:-ng, :], 2, 0)).view(np.uint64).ravel()
+    idx = np.arange(a.size, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = a ^ (idx * np.uint64(0x9E3779B97F4A7C15))
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+        return int(np.sum(z, dtype=np.uint64))

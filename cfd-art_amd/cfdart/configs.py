@@ -16,16 +16,22 @@ Z_EPS = 1e-6
 
 
 def euler_discs(n=200, n_discs=8, r=(0.03, 0.10), bc=(0, 0, 0, 0), seed=1, limiter=L.VANLEER, ny=None):
+    d, p = euler_discs_params(n, n_discs, r, bc, seed, limiter, ny)
+    return d, L.random_discs(d, p)
+
+
+def euler_discs_params(n=200, n_discs=8, r=(0.03, 0.10), bc=(0, 0, 0, 0), seed=1, limiter=L.VANLEER, ny=None):
+    """Descriptor and generator parameters of euler_discs (for device-side generation)."""
     ny = n if ny is None else ny
     d = L.make_desc(L.EULER, n, ny, lx=1.0, ly=ny / n, gamma1=1.4, bc=bc, limiter=limiter)
     p = L.DiscParams()
     p.seed, p.n = seed, n_discs
-    p.cx_lo, p.cx_hi, p.cy_lo, p.cy_hi = 0.1, 0.9, 0.1 * ny / n, 0.9 * ny / n
+    p.cx_lo, p.cx_hi, p.cy_lo, p.cy_hi = 0.1, 0.9, 0.1 * (ny / n), 0.9 * (ny / n)
     p.r_lo, p.r_hi = r
     p.p_lo, p.p_hi, p.rho_lo, p.rho_hi = 5.0, 20.0, 0.5, 2.0
     for i, v in enumerate((1.0, 0.0, 0.0, 1.0)):
         p.ambient[i] = v
-    return d, L.random_discs(d, p)
+    return d, p
 
 
 def euler_200(seed=1):
@@ -37,13 +43,19 @@ def euler_2048(seed=1):
 
 
 def allaire_bubbles(n=1024, n_bubbles=16, seed=1, ny=None, wave_speeds=L.PVRS, limiter=L.MINMOD):
+    d, p = allaire_bubbles_params(n, n_bubbles, seed, ny, wave_speeds, limiter)
+    return d, L.shocked_bubbles(d, p)
+
+
+def allaire_bubbles_params(n=1024, n_bubbles=16, seed=1, ny=None, wave_speeds=L.PVRS, limiter=L.MINMOD):
+    """Descriptor and generator parameters of allaire_bubbles (for device-side generation)."""
     ny = n if ny is None else ny
     d = L.make_desc(L.ALLAIRE, n, ny, lx=1.0, ly=ny / n, gamma1=4.4, pinf1=6e8, gamma2=1.4, pinf2=0.0,
                     bc=(0, 0, 0, 0), limiter=limiter, wave_speeds=wave_speeds, z_lo=Z_EPS, z_hi=1.0 - Z_EPS)
     zw, za = 1.0 - Z_EPS, Z_EPS
     p = L.BubbleParams()
     p.seed, p.n = seed, n_bubbles
-    p.cx_lo, p.cx_hi, p.cy_lo, p.cy_hi = 0.15, 0.95, 0.05 * ny / n, 0.95 * ny / n
+    p.cx_lo, p.cx_hi, p.cy_lo, p.cy_hi = 0.15, 0.95, 0.05 * (ny / n), 0.95 * (ny / n)
     p.r_lo, p.r_hi = 0.02, 0.06
     water = (zw * 1000.0, (1 - zw) * 1.0, 0.0, 0.0, 1e5, zw)
     air = (za * 1000.0, (1 - za) * 1.0, 0.0, 0.0, 1e5, za)
@@ -51,7 +63,7 @@ def allaire_bubbles(n=1024, n_bubbles=16, seed=1, ny=None, wave_speeds=L.PVRS, l
     for i in range(6):
         p.water[i], p.air[i], p.postshock[i] = water[i], air[i], post[i]
     p.shock_x = 0.1
-    return d, L.shocked_bubbles(d, p)
+    return d, p
 
 
 def allaire_1024(seed=1):

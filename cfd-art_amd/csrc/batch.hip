@@ -14,10 +14,12 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <list>
 #include <map>
 #include <string>
 #include <vector>
 
+#include "ic_device.h"
 #include "sweep_device.h"
 
 namespace cfd {
@@ -120,7 +122,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void b
 
 // End of a step for every live grid (E/headers/simulation.hpp:68-70 per grid, plus the
 // step budget), and the queue reset for the next step.
-__global__ void batch_finalize_kernel(BatchGrid* grids, int cap, BatchQueue* q)
+__global__ void batch_finalize_kernel(BatchGrid* grids, int cap, BatchQueue* q, int* status)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
@@ -139,10 +141,12 @@ __global__ void batch_finalize_kernel(BatchGrid* grids, int cap, BatchQueue* q)
     c->t = t;
     c->n = n;
     bg->steps_left -= 1;
-    c->active = (t < c->T) && bg->steps_left > 0;
+    c->active = (t < c->T) && bg->steps_left > 0 && !c->error;
     double mu, mv;
     fold_maxima(c, mu, mv);
     if (c->active) c->dt = dt_from_maxima(c, mu, mv, n, t, bg->dx, bg->dy);
+    status[2 * i] = n;
+    status[2 * i + 1] = (c->active ? 1 : 0) | (c->error ? 2 : 0);
 }
 
 __global__ void batch_reset_queue_kernel(BatchQueue* q, int ntiles_euler, int ntiles_allaire)
@@ -155,7 +159,7 @@ __global__ void batch_reset_queue_kernel(BatchQueue* q, int ntiles_euler, int nt
 }
 
 // After the reduction of a newly admitted grid: its first dt and loop test.
-__global__ void batch_admit_kernel(BatchGrid* bg)
+__global__ void batch_admit_kernel(BatchGrid* bg, int* status)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     Ctrl* c = &bg->ctrl;
@@ -163,19 +167,8 @@ __global__ void batch_admit_kernel(BatchGrid* bg)
     fold_maxima(c, mu, mv);
     c->active = (c->t < c->T) && bg->steps_left > 0;
     if (c->active) c->dt = dt_from_maxima(c, mu, mv, c->n, c->t, bg->dx, bg->dy);
-}
-
-// Frame snapshot: contiguous copy of the interior of buf[0] (SoA planes, pitch removed).
-__global__ void batch_snapshot_kernel(const BatchGrid* bg, double* dst)
-{
-    const int nc = bg->model == ALLAIRE ? 6 : 4;
-    const long long n = (long long)bg->nx * bg->ny;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n * nc;
-         i += (long long)gridDim.x * blockDim.x) {
-        long long k = i / n, c = i % n;
-        long long y = c / bg->nx, x = c % bg->nx;
-        dst[i] = bg->buf[0][k * bg->plane + y * bg->pitch + x];
-    }
+    status[0] = c->n;
+    status[1] = c->active ? 1 : 0;
 }
 
 }  // namespace cfd
@@ -183,6 +176,11 @@ __global__ void batch_snapshot_kernel(const BatchGrid* bg, double* dst)
 using cfd::BatchGrid;
 using cfd::BatchQueue;
 using cfd::Tile;
+
+struct FrameRegion {
+    size_t off, len;  // doubles
+    hipEvent_t done;  // the device-to-host copy out of the region has finished
+};
 
 struct cfdart_batch {
     int cap;
@@ -204,9 +202,14 @@ struct cfdart_batch {
     int nblocks[2];
     int ntiles;
     std::multimap<size_t, double*> pool;  // freed state buffers for reuse, by size
-    double* d_snap;
-    size_t snap_cap;
-    hipEvent_t snap_ev;
+    int* d_status;           // per slot: steps taken, flags (1 running, 2 error)
+    bool tiles_dirty;        // the work-list is rebuilt before the next step
+    void* d_draws;           // initial-condition draws of the admission in progress
+    unsigned long long* d_sum;
+    // frame staging ring in device memory: regions still being copied to the host
+    double* d_ring;
+    size_t ring_cap, ring_head;  // in doubles
+    std::list<FrameRegion> inflight;
 };
 
 namespace {
@@ -315,7 +318,7 @@ int capture_graph(cfdart_batch* b)
     BCHK(hipEventRecord(b->join_ev, b->st2));
     BCHK(hipStreamWaitEvent(b->st, b->join_ev, 0));
     hipLaunchKernelGGL(cfd::batch_finalize_kernel, dim3((b->cap + 255) / 256), dim3(256), 0, b->st, b->d_grids,
-                       b->cap, b->d_q);
+                       b->cap, b->d_q, b->d_status);
     BCHK(hipStreamEndCapture(b->st, &graph));
     hipError_t e = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
@@ -352,12 +355,15 @@ int cfdart_batch_create(int capacity, size_t mem_budget_bytes, cfdart_batch** ou
     b->ntile[0] = b->ntile[1] = 0;
     b->have_graph = false;
     b->ntiles = 0;
-    b->d_snap = nullptr;
-    b->snap_cap = 0;
+    b->d_status = nullptr;
+    b->tiles_dirty = false;
+    b->d_draws = nullptr;
+    b->d_sum = nullptr;
+    b->d_ring = nullptr;
+    b->ring_cap = b->ring_head = 0;
     if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&b->st2, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&b->copy_st, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&b->snap_ev, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&b->join_ev, hipEventDisableTiming) != hipSuccess) {
         delete b;
@@ -365,11 +371,15 @@ int cfdart_batch_create(int capacity, size_t mem_budget_bytes, cfdart_batch** ou
     }
     if (hipMalloc(&b->d_grids, sizeof(BatchGrid) * capacity) != hipSuccess ||
         hipMalloc(&b->d_q, sizeof(BatchQueue)) != hipSuccess ||
-        hipMalloc(&b->d_partials, sizeof(double) * 2 * 1024) != hipSuccess) {
+        hipMalloc(&b->d_partials, sizeof(double) * 2 * 1024) != hipSuccess ||
+        hipMalloc(&b->d_status, sizeof(int) * 2 * capacity) != hipSuccess ||
+        hipMalloc(&b->d_draws, cfd::IC_DRAW_BYTES) != hipSuccess ||
+        hipMalloc(&b->d_sum, sizeof(unsigned long long)) != hipSuccess) {
         cfdart_batch_destroy(b);
         return bfail(CFDART_E_NOMEM, "batch tables");
     }
     (void)hipMemset(b->d_grids, 0, sizeof(BatchGrid) * capacity);
+    (void)hipMemset(b->d_status, 0, sizeof(int) * 2 * capacity);
     (void)hipMemset(b->d_q, 0, sizeof(BatchQueue));
     // persistent grid: as many workgroups as can be resident at once
     int dev = 0, cus = 0, per_cu[2] = {0, 0};
@@ -404,8 +414,11 @@ void cfdart_batch_destroy(cfdart_batch* b)
     if (b->d_grids) (void)hipFree(b->d_grids);
     if (b->d_q) (void)hipFree(b->d_q);
     if (b->d_partials) (void)hipFree(b->d_partials);
-    if (b->d_snap) (void)hipFree(b->d_snap);
-    (void)hipEventDestroy(b->snap_ev);
+    if (b->d_status) (void)hipFree(b->d_status);
+    if (b->d_draws) (void)hipFree(b->d_draws);
+    if (b->d_sum) (void)hipFree(b->d_sum);
+    if (b->d_ring) (void)hipFree(b->d_ring);
+    for (FrameRegion& r : b->inflight) (void)hipEventDestroy(r.done);
     (void)hipEventDestroy(b->fork_ev);
     (void)hipEventDestroy(b->join_ev);
     (void)hipStreamDestroy(b->st2);
@@ -414,20 +427,25 @@ void cfdart_batch_destroy(cfdart_batch* b)
     delete b;
 }
 
-int cfdart_batch_add(cfdart_batch* b, const cfdart_desc* d, const double* host, int splitting, int max_steps,
-                     double T, double cfl, int yH, int xrows, int* slot_out)
+}  // extern "C"
+
+namespace {
+
+// Admission shared by the host-state and the generated-state entry points: budget and slot
+// checks, buffers, the initial state (from host_state, or generated on the device from dp / bp),
+// then compute_dt's reduction and the grid's first dt. The work-list is rebuilt lazily.
+int admit(cfdart_batch* b, const cfdart_desc* d, const double* host, const cfdart_disc_params* dp,
+          const cfdart_bubble_params* bp, int splitting, int max_steps, double T, double cfl, int yH, int xrows,
+          int* slot_out)
 {
-    if (!b || !d || !host || !slot_out) return bfail(CFDART_E_ARG, "null argument");
+    if (!b || !d || !slot_out) return bfail(CFDART_E_ARG, "null argument");
     if (splitting != CFDART_NAIVE && splitting != CFDART_ALTERNATING)
         return bfail(CFDART_E_ARG, "batched grids use naive or alternating splitting");
     if (max_steps < 1 || !(cfl > 0.0)) return bfail(CFDART_E_ARG, "need max_steps >= 1 and cfl > 0");
-    // descriptor validation shared with cfdart_create
-    cfdart_grid* probe = nullptr;
-    (void)probe;
     if (d->model != CFDART_EULER && d->model != CFDART_ALLAIRE) return bfail(CFDART_E_ARG, "bad model");
     if (d->nx < 2 || d->ny < 2 || d->ng != 2 || !(d->dx > 0) || !(d->dy > 0)) return bfail(CFDART_E_ARG, "bad grid");
     if (yH <= 0) yH = 32;
-    if (xrows <= 0) xrows = 4;
+    if (xrows <= 0) xrows = 8;
     const int nc = d->model == CFDART_ALLAIRE ? 6 : 4;
     const int pitch = (d->nx + 31) / 32 * 32;
     const long long plane = (long long)pitch * d->ny;
@@ -469,14 +487,33 @@ int cfdart_batch_add(cfdart_batch* b, const cfdart_desc* d, const double* host, 
     g.ctrl.T = T;
     g.ctrl.cfl = cfl;
 
-    // interior of the host grid -> SoA buf[0]
-    std::vector<double> soa((size_t)plane * nc, 0.0);
-    const int W = d->nx + 2 * d->ng;
-    for (int k = 0; k < nc; ++k)
-        for (int y = 0; y < d->ny; ++y)
-            for (int x = 0; x < d->nx; ++x)
-                soa[(size_t)k * plane + (size_t)y * pitch + x] = host[((size_t)(y + d->ng) * W + (x + d->ng)) * nc + k];
-    BCHK(hipMemcpyAsync(g.buf[0], soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice, b->st));
+    auto release = [&]() {
+        (void)hipStreamSynchronize(b->st);
+        b->pool.insert({one, g.buf[0]});
+        b->pool.insert({one, g.buf[1]});
+    };
+    std::vector<double> soa;
+    if (host) {
+        // interior of the host grid -> SoA buf[0]
+        soa.assign((size_t)plane * nc, 0.0);
+        const int W = d->nx + 2 * d->ng;
+        for (int k = 0; k < nc; ++k)
+            for (int y = 0; y < d->ny; ++y)
+                for (int x = 0; x < d->nx; ++x)
+                    soa[(size_t)k * plane + (size_t)y * pitch + x] = host[((size_t)(y + d->ng) * W + (x + d->ng)) * nc + k];
+        hipError_t e = hipMemcpyAsync(g.buf[0], soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice, b->st);
+        if (e != hipSuccess) {
+            release();
+            return bfail(CFDART_E_HIP, std::string("state upload: ") + hipGetErrorString(e));
+        }
+    } else {
+        std::string err;
+        int rc = cfd::generate_state(*d, dp, bp, g.buf[0], pitch, plane, b->d_draws, b->st, err);
+        if (rc) {
+            release();
+            return bfail(rc, err);
+        }
+    }
     BCHK(hipMemcpyAsync(b->d_grids + slot, &g, sizeof(BatchGrid), hipMemcpyHostToDevice, b->st));
     // first dt: compute_dt's reduction on the new state into the grid's control block
     cfd::GridView v;
@@ -495,7 +532,8 @@ int cfdart_batch_add(cfdart_batch* b, const cfdart_desc* d, const double* host, 
     v.ctrl = &(b->d_grids + slot)->ctrl;
     hipError_t e = cfd::launch_reduce_dt(v, g.model, b->d_partials, b->st);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(cfd::batch_admit_kernel, dim3(1), dim3(64), 0, b->st, b->d_grids + slot);
+        hipLaunchKernelGGL(cfd::batch_admit_kernel, dim3(1), dim3(64), 0, b->st, b->d_grids + slot,
+                           b->d_status + 2 * slot);
         e = hipGetLastError();
     }
     if (e != hipSuccess) return bfail(CFDART_E_HIP, std::string("admission kernels: ") + hipGetErrorString(e));
@@ -504,10 +542,27 @@ int cfdart_batch_add(cfdart_batch* b, const cfdart_desc* d, const double* host, 
     b->live[slot] = 1;
     b->bytes[slot] = need;
     b->used += need;
-    int rc = rebuild_tiles(b);
-    if (rc) return rc;
+    b->tiles_dirty = true;
     *slot_out = slot;
     return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cfdart_batch_add(cfdart_batch* b, const cfdart_desc* d, const double* host, int splitting, int max_steps,
+                     double T, double cfl, int yH, int xrows, int* slot_out)
+{
+    if (!host) return bfail(CFDART_E_ARG, "null host state");
+    return admit(b, d, host, nullptr, nullptr, splitting, max_steps, T, cfl, yH, xrows, slot_out);
+}
+
+int cfdart_batch_add_generated(cfdart_batch* b, const cfdart_desc* d, const cfdart_disc_params* discs,
+                               const cfdart_bubble_params* bubbles, int splitting, int max_steps, double T,
+                               double cfl, int yH, int xrows, int* slot_out)
+{
+    return admit(b, d, nullptr, discs, bubbles, splitting, max_steps, T, cfl, yH, xrows, slot_out);
 }
 
 int cfdart_batch_remove(cfdart_batch* b, int slot)
@@ -519,8 +574,7 @@ int cfdart_batch_remove(cfdart_batch* b, int slot)
     // mark dead on the device before the buffers can be reused
     BCHK(hipMemcpyAsync(&(b->d_grids + slot)->live, &g.live, sizeof(int), hipMemcpyHostToDevice, b->st));
     b->live[slot] = 0;
-    rc = rebuild_tiles(b);
-    if (rc) return rc;
+    b->tiles_dirty = true;
     BCHK(hipStreamSynchronize(b->st));
     size_t one = b->bytes[slot] / 2;
     b->pool.insert({one, g.buf[0]});
@@ -533,6 +587,11 @@ int cfdart_batch_remove(cfdart_batch* b, int slot)
 int cfdart_batch_step(cfdart_batch* b, int nsteps)
 {
     if (!b || nsteps < 0) return bfail(CFDART_E_ARG, "bad argument");
+    if (b->tiles_dirty) {
+        int rc = rebuild_tiles(b);
+        if (rc) return rc;
+        b->tiles_dirty = false;
+    }
     if (b->ntiles == 0) return 0;
     if (!b->have_graph) {
         int rc = capture_graph(b);
@@ -618,12 +677,18 @@ int cfdart_batch_set_tile(cfdart_batch* b, int slot, int yH, int xrows)
     // only the two tile-shape fields change on the device
     BCHK(hipMemcpyAsync(&(b->d_grids + slot)->yH, &g.yH, 2 * sizeof(int), hipMemcpyHostToDevice, b->st));
     BCHK(hipStreamSynchronize(b->st));
-    return rebuild_tiles(b);
+    b->tiles_dirty = true;
+    return 0;
 }
 
 int cfdart_batch_info(cfdart_batch* b, int* live, size_t* bytes, int* tiles)
 {
     if (!b) return bfail(CFDART_E_ARG, "null batch");
+    if (tiles && b->tiles_dirty) {  // the tile count needs the work-list of the live grids
+        int rc = rebuild_tiles(b);
+        if (rc) return rc;
+        b->tiles_dirty = false;
+    }
     int n = 0;
     for (int s = 0; s < b->cap; ++s) n += b->live[s];
     if (live) *live = n;
@@ -636,26 +701,51 @@ int cfdart_batch_frame(cfdart_batch* b, int slot, double* host_dst)
 {
     int rc = check_slot(b, slot);
     if (rc) return rc;
+    if (!host_dst) return bfail(CFDART_E_ARG, "null frame buffer");
     const BatchGrid& g = b->h_grids[slot];
-    const size_t n = (size_t)g.nx * g.ny * (g.model == cfd::ALLAIRE ? 6 : 4);
-    if (n > b->snap_cap) {
+    const int nc = g.model == cfd::ALLAIRE ? 6 : 4;
+    const size_t n = (size_t)g.nx * g.ny * nc;
+    if (n > b->ring_cap / 2) {
+        // grow the ring to hold at least two frames of this size (and 128 Mi doubles)
         BCHK(hipStreamSynchronize(b->st));
         BCHK(hipStreamSynchronize(b->copy_st));
-        if (b->d_snap) BCHK(hipFree(b->d_snap));
-        b->snap_cap = std::max(n, 2 * b->snap_cap);
-        BCHK(hipMalloc(&b->d_snap, sizeof(double) * b->snap_cap));
+        for (FrameRegion& r : b->inflight) (void)hipEventDestroy(r.done);
+        b->inflight.clear();
+        if (b->d_ring) BCHK(hipFree(b->d_ring));
+        b->d_ring = nullptr;
+        b->ring_cap = std::max<size_t>(2 * n, size_t(1) << 27);
+        b->ring_head = 0;
+        if (hipMalloc(&b->d_ring, sizeof(double) * b->ring_cap) != hipSuccess) {
+            b->ring_cap = 0;
+            return bfail(CFDART_E_NOMEM, "frame staging ring");
+        }
     }
-    // the staging buffer must be free: the previous frame's copy has to be done
-    BCHK(hipStreamWaitEvent(b->st, b->snap_ev, 0));
-    hipLaunchKernelGGL(cfd::batch_snapshot_kernel, dim3(1024), dim3(256), 0, b->st, b->d_grids + slot, b->d_snap);
-    BCHK(hipGetLastError());
-    hipEvent_t ready;
+    size_t off = b->ring_head;
+    if (off + n > b->ring_cap) off = 0;
+    // the region must not hold a frame whose copy to the host is still running
+    for (auto it = b->inflight.begin(); it != b->inflight.end();) {
+        if (it->off < off + n && off < it->off + it->len) {
+            BCHK(hipStreamWaitEvent(b->st, it->done, 0));
+            (void)hipEventDestroy(it->done);
+            it = b->inflight.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    // stage the interior planes (contiguous, pitch removed) on the batch stream: the grid's
+    // state after a whole step is buf[0]
+    BCHK(hipMemcpy2DAsync(b->d_ring + off, sizeof(double) * g.nx, g.buf[0], sizeof(double) * g.pitch,
+                          sizeof(double) * g.nx, (size_t)g.ny * nc, hipMemcpyDeviceToDevice, b->st));
+    hipEvent_t ready, done;
     BCHK(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    BCHK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
     BCHK(hipEventRecord(ready, b->st));
     BCHK(hipStreamWaitEvent(b->copy_st, ready, 0));
-    BCHK(hipMemcpyAsync(host_dst, b->d_snap, sizeof(double) * n, hipMemcpyDeviceToHost, b->copy_st));
-    BCHK(hipEventRecord(b->snap_ev, b->copy_st));
+    BCHK(hipMemcpyAsync(host_dst, b->d_ring + off, sizeof(double) * n, hipMemcpyDeviceToHost, b->copy_st));
+    BCHK(hipEventRecord(done, b->copy_st));
     (void)hipEventDestroy(ready);
+    b->inflight.push_back(FrameRegion{off, n, done});
+    b->ring_head = off + n;
     return 0;
 }
 
@@ -663,6 +753,34 @@ int cfdart_batch_frames_done(cfdart_batch* b)
 {
     if (!b) return bfail(CFDART_E_ARG, "null batch");
     BCHK(hipStreamSynchronize(b->copy_st));
+    for (FrameRegion& r : b->inflight) (void)hipEventDestroy(r.done);
+    b->inflight.clear();
+    return 0;
+}
+
+int cfdart_batch_poll(cfdart_batch* b, int* steps, int* flags)
+{
+    if (!b) return bfail(CFDART_E_ARG, "null batch");
+    std::vector<int> st(2 * (size_t)b->cap);
+    BCHK(hipMemcpyAsync(st.data(), b->d_status, sizeof(int) * st.size(), hipMemcpyDeviceToHost, b->st));
+    BCHK(hipStreamSynchronize(b->st));
+    for (int s = 0; s < b->cap; ++s) {
+        if (steps) steps[s] = b->live[s] ? st[2 * s] : 0;
+        if (flags) flags[s] = b->live[s] ? st[2 * s + 1] : 0;
+    }
+    return 0;
+}
+
+int cfdart_batch_checksum(cfdart_batch* b, int slot, unsigned long long* out)
+{
+    int rc = check_slot(b, slot);
+    if (rc) return rc;
+    if (!out) return bfail(CFDART_E_ARG, "null output");
+    const BatchGrid& g = b->h_grids[slot];
+    BCHK(hipMemsetAsync(b->d_sum, 0, sizeof(unsigned long long), b->st));
+    BCHK(cfd::launch_checksum(g.buf[0], g.pitch, g.plane, g.nx, g.ny, g.model == cfd::ALLAIRE ? 6 : 4, b->d_sum, b->st));
+    BCHK(hipMemcpyAsync(out, b->d_sum, sizeof(unsigned long long), hipMemcpyDeviceToHost, b->st));
+    BCHK(hipStreamSynchronize(b->st));
     return 0;
 }
 

@@ -9,6 +9,8 @@
 #include <string>
 #include <vector>
 
+#include "cfdart_host.h"
+#include "ic_device.h"
 #include "sweep_kernels.h"
 
 struct cfdart_grid {
@@ -30,6 +32,7 @@ struct cfdart_grid {
     size_t ev_used;
     double prof_ms[3];
     long long prof_n[3];
+    unsigned long long* d_sum;  // checksum accumulator
 };
 
 namespace {
@@ -190,6 +193,7 @@ int cfdart_create(const cfdart_desc* desc, cfdart_grid** out)
     g->buf[0] = g->buf[1] = nullptr;
     g->ctrl = nullptr;
     g->partials = nullptr;
+    g->d_sum = nullptr;
     g->profiling = 0;
     g->ev_used = 0;
     for (int i = 0; i < 3; ++i) { g->prof_ms[i] = 0.0; g->prof_n[i] = 0; }
@@ -199,7 +203,8 @@ int cfdart_create(const cfdart_desc* desc, cfdart_grid** out)
         return fail(CFDART_E_HIP, "hipStreamCreate failed");
     }
     if (hipMalloc(&g->buf[0], bytes) != hipSuccess || hipMalloc(&g->buf[1], bytes) != hipSuccess ||
-        hipMalloc(&g->ctrl, sizeof(cfd::Ctrl)) != hipSuccess) {
+        hipMalloc(&g->ctrl, sizeof(cfd::Ctrl)) != hipSuccess ||
+        hipMalloc(&g->d_sum, sizeof(unsigned long long)) != hipSuccess) {
         cfdart_destroy(g);
         return fail(CFDART_E_NOMEM, "hipMalloc of the grid state failed");
     }
@@ -228,6 +233,7 @@ void cfdart_destroy(cfdart_grid* g)
     if (g->buf[1]) (void)hipFree(g->buf[1]);
     if (g->ctrl) (void)hipFree(g->ctrl);
     if (g->partials) (void)hipFree(g->partials);
+    if (g->d_sum) (void)hipFree(g->d_sum);
     for (hipEvent_t e : g->ev_pool) (void)hipEventDestroy(e);
     if (g->st) (void)hipStreamDestroy(g->st);
     delete g;
@@ -249,6 +255,56 @@ int cfdart_upload(cfdart_grid* g, const double* host)
     HIPCHK(hipMemcpy(g->buf[0], soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemset(g->ctrl, 0, sizeof(cfd::Ctrl)));
     HIPCHK(hipDeviceSynchronize());
+    return 0;
+}
+
+namespace {
+
+int init_generated(cfdart_grid* g, const cfdart_disc_params* dp, const cfdart_bubble_params* bp)
+{
+    if (!g) return fail(CFDART_E_ARG, "null grid");
+    void* d_draws = nullptr;
+    HIPCHK(hipStreamSynchronize(g->st));
+    HIPCHK(hipMalloc(&d_draws, cfd::IC_DRAW_BYTES));
+    g->cur = 0;
+    std::string err;
+    int rc = cfd::generate_state(g->d, dp, bp, g->buf[0], g->pitch, g->plane, d_draws, g->st, err);
+    (void)hipFree(d_draws);
+    if (rc) return fail(rc, err);
+    HIPCHK(hipMemsetAsync(g->ctrl, 0, sizeof(cfd::Ctrl), g->st));
+    HIPCHK(hipStreamSynchronize(g->st));
+    return 0;
+}
+
+}  // namespace
+
+int cfdart_init_random_discs(cfdart_grid* g, const cfdart_disc_params* p)
+{
+    if (!p) return fail(CFDART_E_ARG, "null parameters");
+    return init_generated(g, p, nullptr);
+}
+
+int cfdart_init_shocked_bubbles(cfdart_grid* g, const cfdart_bubble_params* p)
+{
+    if (!p) return fail(CFDART_E_ARG, "null parameters");
+    return init_generated(g, nullptr, p);
+}
+
+int cfdart_checksum(cfdart_grid* g, unsigned long long* out)
+{
+    if (!g || !out) return fail(CFDART_E_ARG, "null argument");
+    HIPCHK(hipMemsetAsync(g->d_sum, 0, sizeof(unsigned long long), g->st));
+    HIPCHK(cfd::launch_checksum(g->buf[g->cur], g->pitch, g->plane, g->d.nx, g->d.ny, g->nc, g->d_sum, g->st));
+    HIPCHK(hipMemcpyAsync(out, g->d_sum, sizeof(unsigned long long), hipMemcpyDeviceToHost, g->st));
+    HIPCHK(hipStreamSynchronize(g->st));
+    return 0;
+}
+
+int cfdart_frame(cfdart_grid* g, double* host_dst)
+{
+    if (!g || !host_dst) return fail(CFDART_E_ARG, "null argument");
+    HIPCHK(hipMemcpy2DAsync(host_dst, sizeof(double) * g->d.nx, g->buf[g->cur], sizeof(double) * g->pitch,
+                            sizeof(double) * g->d.nx, (size_t)g->d.ny * g->nc, hipMemcpyDeviceToHost, g->st));
     return 0;
 }
 

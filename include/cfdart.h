@@ -114,6 +114,16 @@ int cfdart_set_strip_rows(cfdart_grid* g, int rows);
 /* Tile shape: rows per y-sweep strip (default 32) and rows per x-sweep wavefront (default 4). */
 int cfdart_set_tile(cfdart_grid* g, int y_strip_rows, int x_rows_per_wave);
 
+/* Checksum of the current interior state: the sum mod 2^64, over every value, of a 64-bit mix
+ * of its bit pattern and its position in [component][y][x] order. Equal states give equal
+ * checksums whatever their device layout; used to compare a grid run alone with the same grid
+ * in a batch. Synchronous. No reference counterpart. */
+int cfdart_checksum(cfdart_grid* g, unsigned long long* out);
+/* Frame: asynchronous copy of the current interior state to host_dst (pinned memory, ncomp SoA
+ * planes of nx*ny doubles) on the grid's stream; complete after cfdart_sync. The state the
+ * reference's output (euler_io.cpp:1142-1261) would write, without its ghost frame. */
+int cfdart_frame(cfdart_grid* g, double* host_dst);
+
 #ifdef __cplusplus
 }
 #endif

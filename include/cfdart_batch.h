@@ -11,6 +11,8 @@
  * tables, without re-capturing the graph.
  *
  * Splitting schemes: CFDART_NAIVE and CFDART_ALTERNATING (two sweeps per step).
+ * Admissions, removals and tile changes take effect at the next cfdart_batch_step, which
+ * rebuilds the work-list once for all of them.
  * Conventions as in cfdart.h: 0 or a negative CFDART_E_* code; one host thread per batch.
  */
 #ifndef CFDART_BATCH_H
@@ -19,6 +21,7 @@
 #include <stddef.h>
 
 #include "cfdart.h"
+#include "cfdart_host.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -39,6 +42,12 @@ void cfdart_batch_destroy(cfdart_batch* b);
 int cfdart_batch_add(cfdart_batch* b, const cfdart_desc* desc, const double* host_state, int splitting,
                      int max_steps, double T, double cfl, int y_strip_rows, int x_rows_per_wave,
                      int* slot_out);
+/* As cfdart_batch_add with the initial state generated on the device from exactly one of
+ * discs (Euler, cfdart_ic_random_discs) or bubbles (Allaire, cfdart_ic_shocked_bubbles);
+ * the other is NULL. Bitwise the state the host generator would upload. */
+int cfdart_batch_add_generated(cfdart_batch* b, const cfdart_desc* desc, const cfdart_disc_params* discs,
+                               const cfdart_bubble_params* bubbles, int splitting, int max_steps, double T,
+                               double cfl, int y_strip_rows, int x_rows_per_wave, int* slot_out);
 /* Removes a grid (finished or not) and frees its device memory for reuse. */
 int cfdart_batch_remove(cfdart_batch* b, int slot);
 /* Advances every live grid by nsteps steps (grids that have finished stay unchanged). */
@@ -47,17 +56,26 @@ int cfdart_batch_step(cfdart_batch* b, int nsteps);
 int cfdart_batch_download(cfdart_batch* b, int slot, double* host_state);
 /* Loop state of a grid: t, steps taken, and whether it is still running; synchronous. */
 int cfdart_batch_status(cfdart_batch* b, int slot, double* t, int* n, int* active);
+/* Loop state of every slot after the work queued so far (synchronous, one small copy):
+ * steps[slot] = steps taken, flags[slot] = 1 while running | 2 if an Allaire state failed the
+ * physical-state test (the grid then stops); 0 for empty slots. Arrays of the batch capacity;
+ * either may be NULL. */
+int cfdart_batch_poll(cfdart_batch* b, int* steps, int* flags);
+/* Checksum of a grid's current state, as cfdart_checksum; synchronous. */
+int cfdart_batch_checksum(cfdart_batch* b, int slot, unsigned long long* out);
 /* Fallback counters of a grid (see CFDART_CNT_*), 4 values; synchronous. */
 int cfdart_batch_stats(cfdart_batch* b, int slot, long long* counters);
 int cfdart_batch_sync(cfdart_batch* b);
 /* Changes a grid's tile shape (used by the autotuner between steps). */
 int cfdart_batch_set_tile(cfdart_batch* b, int slot, int y_strip_rows, int x_rows_per_wave);
-/* Live grids, device bytes held by grid state, tiles in the work-list. */
+/* Live grids, device bytes held by grid state, tiles in the work-list (each may be NULL; asking
+ * for the tile count rebuilds a work-list left stale by admissions or removals). */
 int cfdart_batch_info(cfdart_batch* b, int* live, size_t* bytes, int* tiles);
-/* Frame snapshot: device-to-device copy of a grid's state into a staging buffer on the batch
- * stream, then an asynchronous copy to host_dst (pinned memory, at least the state's interior
- * size: ncomp * nx * ny doubles, SoA planes) on the copy stream. Completion: cfdart_batch_sync
- * or cfdart_batch_frames_done. */
+/* Frame snapshot: device-to-device copy of a grid's state into a staging ring in device memory
+ * on the batch stream, then an asynchronous copy to host_dst (pinned memory, at least the
+ * state's interior size: ncomp * nx * ny doubles, SoA planes) on the copy stream, so stepping
+ * continues while frames drain. The batch stream waits only when the ring wraps onto a frame
+ * still being copied. Completion: cfdart_batch_sync or cfdart_batch_frames_done. */
 int cfdart_batch_frame(cfdart_batch* b, int slot, double* host_dst);
 int cfdart_batch_frames_done(cfdart_batch* b);
 void* cfdart_batch_stream(cfdart_batch* b);

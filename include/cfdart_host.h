@@ -1,7 +1,8 @@
 /*
- * cfdart host side: initial conditions and frame output of the reference's problem classes,
- * restated in C++ (no GPU work). These produce and consume the host state format of
- * cfdart.h (the reference's grid[i][j](k) with its ghost frame).
+ * cfdart initial conditions: the reference's problem-class initial states, restated in C++.
+ * The host generators produce the host state format of cfdart.h (the reference's grid[i][j](k)
+ * with its ghost frame); the device generators (cfdart_init_*) write the same interior state,
+ * bitwise, straight into a grid's device memory.
  */
 #ifndef CFDART_HOST_H
 #define CFDART_HOST_H
@@ -49,6 +50,14 @@ int cfdart_ic_shocked_bubbles(const cfdart_desc* d, const cfdart_bubble_params* 
  * (allaire_2d/sourcecode/allaire_diffuse_io.cpp:332-360); primitive vectors of the model. */
 int cfdart_ic_planar(const cfdart_desc* d, const double* W_under, const double* W_over, double a,
                      double b, double c, double* host_state);
+
+/* Device-side generation: the state cfdart_ic_random_discs / cfdart_ic_shocked_bubbles would
+ * produce, written straight into the grid's device buffer (no host loop over cells, no
+ * host-to-device copy of the state; the reference builds ICs on the host,
+ * euler_equations/euler_simulation_code/sourcecode/euler_io.cpp:282-433). Resets the grid's
+ * time-loop state as cfdart_upload does; synchronous. */
+int cfdart_init_random_discs(cfdart_grid* g, const cfdart_disc_params* p);
+int cfdart_init_shocked_bubbles(cfdart_grid* g, const cfdart_bubble_params* p);
 
 /* Primitive -> conserved of the model (euler_equations/.../headers/misc_euler.hpp:82-97,
  * allaire_2d/headers/misc.hpp:149-163). */

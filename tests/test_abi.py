@@ -15,7 +15,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_functions():
     names = set()
-    for h in ("cfdart.h", "cfdart_host.h", "cfdart_batch.h"):
+    headers = sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h"))
+    assert {"cfdart.h", "cfdart_host.h", "cfdart_batch.h", "cfdart_serve.h"} <= set(headers)
+    for h in headers:
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         for m in re.finditer(r"\b(cfdart_[a-z0-9_]+)\s*\(", text):
