@@ -185,7 +185,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--cpu-steps", type=int, default=12, help="oracle steps for the CPU baseline (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=160,
+                    help="oracle steps of euler_2048 for the CPU baseline, about 10 s on 16 threads (0: skip)")
     ap.add_argument("--strip-rows", type=int, default=0)
     ap.add_argument("--tile", default="", help="Y_STRIP_ROWS,X_ROWS_PER_WAVE: fixed tile shape, no autotuning")
     ap.add_argument("--workload", default="euler_2048", choices=["euler_2048", "batched", "served"])

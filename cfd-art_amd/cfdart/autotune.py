@@ -30,15 +30,31 @@ def _case(model, n):
     return configs.allaire_bubbles(n, 16, seed=7)
 
 
+def _around(best, lo, step):
+    """Refinement candidates between best/2 and 2*best (the coarse grid's neighbours)."""
+    return sorted({max(lo, int(round(best * f / step)) * step)
+                   for f in (0.5, 0.625, 0.75, 0.875, 1.0, 1.125, 1.25, 1.5, 1.75, 2.0)})
+
+
 def tune_one(model, n, reps=5, log=None, case=None):
+    """Coarse search over powers of two, then a finer search around the best of each axis: the
+    number of wavefronts a shape launches against the chip's resident-wave slots (256 CUs x
+    4 SIMDs x waves per SIMD) decides how full the last round of waves is, which powers of two
+    alone do not resolve."""
     d, h = case if case is not None else _case(model, n)
     tx = {xr: L.time_tile(d, 32, xr, reps, h)[0] for xr in X_ROWS}
     ty = {yh: L.time_tile(d, yh, 4, reps, h)[1] for yh in Y_STRIPS}
+    for xr in _around(min(tx, key=tx.get), 1, 1):
+        if xr not in tx:
+            tx[xr] = L.time_tile(d, 32, xr, reps, h)[0]
+    for yh in _around(min(ty, key=ty.get), 4, 2):
+        if yh not in ty:
+            ty[yh] = L.time_tile(d, yh, 4, reps, h)[1]
     bx = min(tx, key=tx.get)
     by = min(ty, key=ty.get)
     if log:
-        log(f"autotune model={model} n={n}: x {{{', '.join(f'{k}:{v:.1f}us' for k, v in tx.items())}}} -> {bx}; "
-            f"y {{{', '.join(f'{k}:{v:.1f}us' for k, v in ty.items())}}} -> {by}")
+        log(f"autotune model={model} n={n}: x {{{', '.join(f'{k}:{v:.1f}us' for k, v in sorted(tx.items()))}}} -> {bx}; "
+            f"y {{{', '.join(f'{k}:{v:.1f}us' for k, v in sorted(ty.items()))}}} -> {by}")
     return {"y_strip_rows": by, "x_rows_per_wave": bx, "x_us": tx[bx], "y_us": ty[by]}
 
 

@@ -38,8 +38,6 @@ template <int M> __device__ __forceinline__ void side_from_left(const Side<M>& s
     o.ok = lane_from_left_i(s.ok ? 1 : 0) != 0;
 }
 
-// Ghost index of the reference's pre_sweep: interior index i of an axis of n cells, with
-// i in [-2, n+2) mapped to its mirror. Indices further out (dead lanes) are clamped in range.
 // Workgroups go to the 8 XCDs (each with its own L2) round-robin in order of linear id.
 // xcd_order maps the linear id b of n workgroups to a tile order in which each XCD receives
 // one contiguous run of tiles (the last n % 8 workgroups keep their id), so tiles that
@@ -52,6 +50,8 @@ __device__ __forceinline__ int xcd_order(int b, int n)
     return (b % NXCD) * q + b / NXCD;
 }
 
+// Ghost index of the reference's pre_sweep: interior index i of an axis of n cells, with
+// i in [-2, n+2) mapped to its mirror. Indices further out (dead lanes) are clamped in range.
 __device__ __forceinline__ int mirror(int i, int n)
 {
     int r = i < 0 ? -1 - i : (i >= n ? 2 * n - 1 - i : i);
@@ -182,16 +182,27 @@ __device__ __forceinline__ void x_tile(const GridView& g, double dt, int tx, int
         cell_stage<M>(g.mat, g.opt.limiter, s, P, Wm, Wp, cs);
 
         // left neighbour's data for this lane's left face
-        Prim<M> Pa;
-#pragma unroll
-        for (int k = 0; k < NC; ++k) Pa.W[k] = Wm[k];
-        Pa.c = lane_from_left(P.c);
-        double Ua[NC];
-#pragma unroll
-        for (int k = 0; k < NC; ++k) Ua[k] = lane_from_left(U[k]);
         CellStage<M> a;
         a.fR = lane_from_left_i(cs.fR ? 1 : 0) != 0;
         side_from_left<M>(cs.R, a.R);
+        // the neighbour's sound speed and raw state feed only the per-face fallbacks (face_stage
+        // reads them when a slope or an evolved state is unphysical); they cross lanes only when
+        // some lane of the wavefront needs them (the ballot keeps the branch wave-uniform, so
+        // every lane takes part in the cross-lane moves)
+        Prim<M> Pa;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) Pa.W[k] = Wm[k];
+        double Ua[NC];
+        const bool need = !(a.fR && cs.fL) || !(a.R.ok && cs.L.ok);
+        if (__builtin_amdgcn_ballot_w64(need)) {
+            Pa.c = lane_from_left(P.c);
+#pragma unroll
+            for (int k = 0; k < NC; ++k) Ua[k] = lane_from_left(U[k]);
+        } else {
+            Pa.c = 0.0;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) Ua[k] = 0.0;
+        }
 
         FaceOut<M> fo;
         int fl = face_stage<M>(g.mat, g.opt, s, Pa, Ua, a, P, U, cs, fo);
