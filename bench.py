@@ -32,9 +32,9 @@ def log(msg):
     print(f"[bench {time.time() - _T0:7.1f}s] {msg}", file=sys.stderr, flush=True)
 
 
-def read_traffic(kernel):
+def read_traffic(workload, kernel):
     """HBM bytes per launch of a kernel from the committed rocprofv3 PMC summary, or None."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic_euler_2048.txt")
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{workload}.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -189,7 +189,7 @@ def main():
                     help="oracle steps of euler_2048 for the CPU baseline, about 10 s on 16 threads (0: skip)")
     ap.add_argument("--strip-rows", type=int, default=0)
     ap.add_argument("--tile", default="", help="Y_STRIP_ROWS,X_ROWS_PER_WAVE: fixed tile shape, no autotuning")
-    ap.add_argument("--workload", default="euler_2048", choices=["euler_2048", "batched", "served"])
+    ap.add_argument("--workload", default="euler_2048", choices=["euler_2048", "allaire_1024", "batched", "served"])
     ap.add_argument("--serve-window", type=float, default=10.0, help="seconds of the arrival trace to replay")
     ap.add_argument("--grids", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=500)
@@ -222,45 +222,58 @@ def main():
     if args.workload == "served":
         print(json.dumps(bench_served(args)), flush=True)
         return
-    d, h = cfdart.configs.euler_2048()
+    single = {
+        "euler_2048": dict(cfg=cfdart.configs.euler_2048, model=cfdart.EULER, cfl=0.8, nc=4, tag="EULER",
+                           metric="cell-updates/s (2048^2 FP64 Euler, MUSCL-Hancock + HLLC)",
+                           data="synthetic (seeded disc generator)",
+                           config={"discs": 64, "bc": "reflective L/R, transmissive B/T", "limiter": "van Leer",
+                                   "riemann": "HLLC"}),
+        "allaire_1024": dict(cfg=cfdart.configs.allaire_1024, model=cfdart.ALLAIRE, cfl=0.5, nc=6, tag="ALLAIRE",
+                             metric="cell-updates/s (1024^2 FP64 Allaire five-equation, MUSCL-Hancock + PVRS-HLLC)",
+                             data="synthetic (seeded bubble generator, planar shock)",
+                             config={"bubbles": 16, "bc": "transmissive", "limiter": "minmod",
+                                     "riemann": "PVRS-HLLC", "z_clip": [1e-6, 1 - 1e-6]}),
+    }[args.workload]
+    d, h = single["cfg"]()
+    cfl = single["cfl"]
     cells = d.nx * d.ny
     if args.tile:
         yh, xr = (int(v) for v in args.tile.split(","))
         tuned = {"y_strip_rows": yh, "x_rows_per_wave": xr}
     else:
-        tuned = cfdart.autotune.tune_one(cfdart.EULER, 2048, log=log, case=(d, h))
+        tuned = cfdart.autotune.tune_one(single["model"], d.nx, log=log, case=(d, h))
     g = cfdart.Grid(d)
     g.set_tile(tuned["y_strip_rows"], tuned["x_rows_per_wave"])
     if args.strip_rows:
         g.set_strip_rows(args.strip_rows)
     g.upload(h)
-    log(f"euler_2048: warm-up {args.warmup} steps, timed {args.steps} steps")
-    g.run(cfdart.ALTERNATING, args.warmup, cfl=0.8)
+    log(f"{args.workload}: warm-up {args.warmup} steps, timed {args.steps} steps")
+    g.run(cfdart.ALTERNATING, args.warmup, cfl=cfl)
     g.set_profiling(True)
     g.sync()
     t0 = time.perf_counter()
-    g.run(cfdart.ALTERNATING, args.steps, cfl=0.8, sync=False)
+    g.run(cfdart.ALTERNATING, args.steps, cfl=cfl, sync=False)
     g.sync()
     el = time.perf_counter() - t0
     prof = g.profile()
     g.set_profiling(False)
-    t_sim, n_sim = g.run(cfdart.ALTERNATING, 0, cfl=0.8)
+    t_sim, n_sim = g.run(cfdart.ALTERNATING, 0, cfl=cfl)
     counters = g.stats()
 
-    bytes_per_launch = 64.0 * cells  # one FP64 read + write of the 4-component state per cell
+    bytes_per_launch = 16.0 * single["nc"] * cells  # one FP64 read + write of the state per cell
     per_kernel = {}
     for axis, (ms, nl) in prof.items():
         if nl:
             per_kernel[axis] = {"avg_us": 1e3 * ms / nl, "launches": nl,
                                 "gbs": bytes_per_launch / (ms / nl * 1e-3) / 1e9}
     dom = max(per_kernel, key=lambda a: per_kernel[a]["avg_us"] * per_kernel[a]["launches"])
-    kname = {"x": "sweep_x_kernel<EULER>", "y": "sweep_y_kernel<EULER>"}[dom]
+    kname = f"sweep_{dom}_kernel<{single['tag']}>"
     achieved = per_kernel[dom]["gbs"]
-    traffic = read_traffic(dom)
+    traffic = read_traffic(args.workload, dom)
 
     value = cells * args.steps / el
     out = {
-        "metric": "cell-updates/s (2048^2 FP64 Euler, MUSCL-Hancock + HLLC)",
+        "metric": single["metric"],
         "value": value,
         "unit": "cell-updates/s",
         "n_gpus": 1,
@@ -271,11 +284,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded disc generator)",
-        "config": {"workload": "euler_2048", "nx": d.nx, "ny": d.ny, "discs": 64,
-                   "bc": "reflective L/R, transmissive B/T", "limiter": "van Leer",
-                   "riemann": "HLLC", "splitting": "alternating", "cfl": 0.8,
-                   "sweeps_per_step": 2, "t_end": t_sim, "steps_total": n_sim},
+        "data": single["data"],
+        "config": dict({"workload": args.workload, "nx": d.nx, "ny": d.ny}, **single["config"],
+                       splitting="alternating", cfl=cfl, sweeps_per_step=2, t_end=t_sim, steps_total=n_sim),
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
@@ -283,7 +294,7 @@ def main():
         "tile": {"y_strip_rows": tuned["y_strip_rows"], "x_rows_per_wave": tuned["x_rows_per_wave"]},
         "fallback_counters": counters,
     }
-    if args.cpu_steps > 0:
+    if args.cpu_steps > 0 and args.workload == "euler_2048":
         out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
     print(json.dumps(out), flush=True)
 

@@ -22,20 +22,35 @@ __device__ __forceinline__ double lane_from_right(double x)  // lane i receives 
     int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x130, 0xf, 0xf, false);
     return __hiloint2double(hi, lo);
 }
-__device__ __forceinline__ int lane_from_left_i(int x)
+// As above, but a lane without a source lane (lane 0 shifting right, lane 63 shifting left)
+// keeps its own value instead of 0 (bound_ctrl off: the DPP move leaves such lanes unwritten).
+__device__ __forceinline__ double lane_from_left_or_self(double x)
 {
-    return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false);
+    int lo = __builtin_amdgcn_update_dpp(__double2loint(x), __double2loint(x), 0x138, 0xf, 0xf, false);
+    int hi = __builtin_amdgcn_update_dpp(__double2hiint(x), __double2hiint(x), 0x138, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_from_right_or_self(double x)
+{
+    int lo = __builtin_amdgcn_update_dpp(__double2loint(x), __double2loint(x), 0x130, 0xf, 0xf, false);
+    int hi = __builtin_amdgcn_update_dpp(__double2hiint(x), __double2hiint(x), 0x130, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ int lane_from_left_or_self_i(int x)
+{
+    return __builtin_amdgcn_update_dpp(x, x, 0x138, 0xf, 0xf, false);
 }
 
+// The right-hand side of the left neighbour's cell stage; lane 0 takes its own.
 template <int M> __device__ __forceinline__ void side_from_left(const Side<M>& s, Side<M>& o)
 {
 #pragma unroll
-    for (int k = 0; k < Model<M>::NC; ++k) o.U[k] = lane_from_left(s.U[k]);
-    o.rho = lane_from_left(s.rho);
-    o.u = lane_from_left(s.u);
-    o.p = lane_from_left(s.p);
-    o.c = lane_from_left(s.c);
-    o.ok = lane_from_left_i(s.ok ? 1 : 0) != 0;
+    for (int k = 0; k < Model<M>::NC; ++k) o.U[k] = lane_from_left_or_self(s.U[k]);
+    o.rho = lane_from_left_or_self(s.rho);
+    o.u = lane_from_left_or_self(s.u);
+    o.p = lane_from_left_or_self(s.p);
+    o.c = lane_from_left_or_self(s.c);
+    o.ok = lane_from_left_or_self_i(s.ok ? 1 : 0) != 0;
 }
 
 // Workgroups go to the 8 XCDs (each with its own L2) round-robin in order of linear id.
@@ -123,7 +138,7 @@ __device__ __forceinline__ bool sweep_dt(const GridView& g, double dt_arg, doubl
 template <int M>
 __device__ __forceinline__ void update_cell(const GridView& g, double r, const double* U,
                                            const double* FL, const double* FR, double usL, double usR,
-                                           double zsL, double zsR, double* Un, bool& bad)
+                                           double zsL, double zsR, double* Un, bool& bad, bool checked = true)
 {
     constexpr int NC = Model<M>::NC;
 #pragma unroll
@@ -132,7 +147,7 @@ __device__ __forceinline__ void update_cell(const GridView& g, double r, const d
     if (M == ALLAIRE) {
         Un[5] = zupdate(r, U[5], usL, usR, zsL, zsR, g.opt.z_lo, g.opt.z_hi);
         snap(Un);
-        bad = !allaire_physical_screened(Un);
+        bad = checked && !allaire_physical_screened(Un);
     }
 }
 
@@ -177,32 +192,23 @@ __device__ __forceinline__ void x_tile(const GridView& g, double dt, int tx, int
         P.from_cons(g.mat, U);
         double Wm[NC], Wp[NC];
 #pragma unroll
-        for (int k = 0; k < NC; ++k) { Wm[k] = lane_from_left(P.W[k]); Wp[k] = lane_from_right(P.W[k]); }
+        for (int k = 0; k < NC; ++k) { Wm[k] = lane_from_left_or_self(P.W[k]); Wp[k] = lane_from_right_or_self(P.W[k]); }
         CellStage<M> cs;
         cell_stage<M>(g.mat, g.opt.limiter, s, P, Wm, Wp, cs);
 
-        // left neighbour's data for this lane's left face
-        CellStage<M> a;
-        a.fR = lane_from_left_i(cs.fR ? 1 : 0) != 0;
-        side_from_left<M>(cs.R, a.R);
-        // the neighbour's sound speed and raw state feed only the per-face fallbacks (face_stage
-        // reads them when a slope or an evolved state is unphysical); they cross lanes only when
-        // some lane of the wavefront needs them (the ballot keeps the branch wave-uniform, so
-        // every lane takes part in the cross-lane moves)
+        // left neighbour's data for this lane's left face (lane 0, which has no left neighbour
+        // and whose face is not used, takes its own cell stage: a physical state, so it sends
+        // the wavefront into no fallback branch)
         Prim<M> Pa;
 #pragma unroll
         for (int k = 0; k < NC; ++k) Pa.W[k] = Wm[k];
+        Pa.c = lane_from_left(P.c);
         double Ua[NC];
-        const bool need = !(a.fR && cs.fL) || !(a.R.ok && cs.L.ok);
-        if (__builtin_amdgcn_ballot_w64(need)) {
-            Pa.c = lane_from_left(P.c);
 #pragma unroll
-            for (int k = 0; k < NC; ++k) Ua[k] = lane_from_left(U[k]);
-        } else {
-            Pa.c = 0.0;
-#pragma unroll
-            for (int k = 0; k < NC; ++k) Ua[k] = 0.0;
-        }
+        for (int k = 0; k < NC; ++k) Ua[k] = lane_from_left(U[k]);
+        CellStage<M> a;
+        a.fR = lane_from_left_or_self_i(cs.fR ? 1 : 0) != 0;
+        side_from_left<M>(cs.R, a.R);
 
         FaceOut<M> fo;
         int fl = face_stage<M>(g.mat, g.opt, s, Pa, Ua, a, P, U, cs, fo);
@@ -213,11 +219,13 @@ __device__ __forceinline__ void x_tile(const GridView& g, double dt, int tx, int
         double usr = lane_from_right(fo.ustar);
         double zsr = M == ALLAIRE ? lane_from_right(fo.zstar) : 0.0;
 
+        // lanes 0 and 63 (halo lanes without a neighbour on one side) update from incomplete
+        // fluxes and are not stored: the repair and the physical-state test skip them
         double Un[NC];
         bool bad;
-        update_cell<M>(g, r, U, fo.F, Fr, fo.ustar, usr, fo.zstar, zsr, Un, bad);
+        update_cell<M>(g, r, U, fo.F, Fr, fo.ustar, usr, fo.zstar, zsr, Un, bad, store);
         bool rep = false;
-        if (M == EULER) rep = euler_repair(g.mat, U, Un);
+        if (M == EULER && store) rep = euler_repair(g.mat, U, Un);
 
         if (face_counted) { tl.slope += fl & 1; tl.evolve += (fl >> 1) & 1; }
         if (store) {

@@ -16,8 +16,10 @@
 
 namespace cfd {
 
+// The Allaire x sweep runs at 3 wavefronts per SIMD (at most 168 VGPRs; a few registers spill),
+// which measured faster than the 2 its natural 176-180 VGPRs allow.
 template <int M, bool LAST>
-__global__ __launch_bounds__(256) void sweep_x_kernel(GridView g, double dt_arg, double dt_factor, int use_ctrl,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M == ALLAIRE ? 3 : 1))) void sweep_x_kernel(GridView g, double dt_arg, double dt_factor, int use_ctrl,
                                                      int xrows)
 {
     // tiles in row-major order: neighbours along x (which share halo columns) are consecutive
