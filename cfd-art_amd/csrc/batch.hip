@@ -120,7 +120,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void b
     }
 }
 
-// End of a step for every live grid (E/headers/simulation.hpp:68-70 per grid, plus the
+// End of a step for every live grid (E/headers/simulation.hpp:56-58 per grid, plus the
 // step budget), and the queue reset for the next step.
 __global__ void batch_finalize_kernel(BatchGrid* grids, int cap, BatchQueue* q, int* status)
 {
