@@ -10,6 +10,7 @@ TRANSMISSIVE, REFLECTIVE = 0, 1
 MINMOD, VANLEER = 0, 1
 GARRICK, PVRS = 0, 1
 NAIVE, STRANG, ALTERNATING = 0, 1, 2
+MUSCL_HANCOCK, GODUNOV = 0, 1
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 library_path = os.path.join(os.path.dirname(_HERE), "libcfdart.so")
@@ -28,6 +29,7 @@ class Desc(C.Structure):
         ("gamma1", C.c_double), ("pinf1", C.c_double), ("gamma2", C.c_double), ("pinf2", C.c_double),
         ("bc", C.c_int * 4), ("limiter", C.c_int), ("wave_speeds", C.c_int),
         ("colsweep_uses_dx", C.c_int), ("z_lo", C.c_double), ("z_hi", C.c_double),
+        ("flux_solver", C.c_int),
     ]
 
 
@@ -142,9 +144,12 @@ def ncomp(model):
 
 def make_desc(model=EULER, nx=200, ny=200, x0=0.0, y0=0.0, lx=1.0, ly=1.0, gamma1=1.4, pinf1=0.0,
               gamma2=1.4, pinf2=0.0, bc=(0, 0, 0, 0), limiter=MINMOD, wave_speeds=GARRICK,
-              colsweep_uses_dx=1, z_lo=0.0, z_hi=1.0):
+              colsweep_uses_dx=1, z_lo=0.0, z_hi=1.0, flux_solver=0, ng=None):
+    """flux_solver: MUSCL_HANCOCK (ghost width 2) or GODUNOV (ghost width 1, as the reference)."""
     d = Desc()
-    d.model, d.nx, d.ny, d.ng = model, nx, ny, 2
+    d.model, d.nx, d.ny = model, nx, ny
+    d.ng = ng if ng is not None else (1 if flux_solver == GODUNOV else 2)
+    d.flux_solver = flux_solver
     d.x0, d.y0, d.dx, d.dy = x0, y0, lx / nx, ly / ny
     d.gamma1, d.pinf1, d.gamma2, d.pinf2 = gamma1, pinf1, gamma2, pinf2
     for i in range(4):

@@ -444,6 +444,7 @@ int admit(cfdart_batch* b, const cfdart_desc* d, const double* host, const cfdar
     if (max_steps < 1 || !(cfl > 0.0)) return bfail(CFDART_E_ARG, "need max_steps >= 1 and cfl > 0");
     if (d->model != CFDART_EULER && d->model != CFDART_ALLAIRE) return bfail(CFDART_E_ARG, "bad model");
     if (d->nx < 2 || d->ny < 2 || d->ng != 2 || !(d->dx > 0) || !(d->dy > 0)) return bfail(CFDART_E_ARG, "bad grid");
+    if (d->flux_solver != CFDART_MUSCL_HANCOCK) return bfail(CFDART_E_ARG, "batched grids use MUSCL-Hancock");
     if (yH <= 0) yH = 32;
     if (xrows <= 0) xrows = 8;
     const int nc = d->model == CFDART_ALLAIRE ? 6 : 4;
@@ -476,6 +477,7 @@ int admit(cfdart_batch* b, const cfdart_desc* d, const double* host, const cfdar
     g.opt.wave_speeds = d->wave_speeds;
     g.opt.z_lo = d->z_lo;
     g.opt.z_hi = d->z_hi;
+    g.opt.godunov = 0;
     g.dx = d->dx;
     g.dy = d->dy;
     g.dx_half_col = d->colsweep_uses_dx ? d->dx : d->dy;

@@ -56,7 +56,12 @@ int check_desc(const cfdart_desc* d)
     if (!d) return fail(CFDART_E_ARG, "null descriptor");
     if (d->model != CFDART_EULER && d->model != CFDART_ALLAIRE) return fail(CFDART_E_ARG, "model must be 0 (Euler) or 1 (Allaire)");
     if (d->nx < 2 || d->ny < 2) return fail(CFDART_E_ARG, "nx and ny must be >= 2");
-    if (d->ng != 2) return fail(CFDART_E_ARG, "ng must be 2 (MUSCL-Hancock stencil)");
+    if (d->flux_solver != CFDART_MUSCL_HANCOCK && d->flux_solver != CFDART_GODUNOV)
+        return fail(CFDART_E_ARG, "flux_solver must be 0 (MUSCL-Hancock) or 1 (Godunov)");
+    if (d->flux_solver == CFDART_MUSCL_HANCOCK && d->ng != 2)
+        return fail(CFDART_E_ARG, "ng must be 2 for MUSCL-Hancock (its stencil)");
+    if (d->flux_solver == CFDART_GODUNOV && d->ng != 1 && d->ng != 2)
+        return fail(CFDART_E_ARG, "ng must be 1 or 2 for Godunov");
     if (!(d->dx > 0.0) || !(d->dy > 0.0)) return fail(CFDART_E_ARG, "dx and dy must be positive");
     if (!(d->gamma1 > 1.0) || (d->model == CFDART_ALLAIRE && !(d->gamma2 > 1.0)))
         return fail(CFDART_E_ARG, "gamma must be > 1");
@@ -86,6 +91,7 @@ cfd::GridView view(const cfdart_grid* g)
     v.opt.wave_speeds = g->d.wave_speeds;
     v.opt.z_lo = g->d.z_lo;
     v.opt.z_hi = g->d.z_hi;
+    v.opt.godunov = g->d.flux_solver == CFDART_GODUNOV ? 1 : 0;
     v.dx = g->d.dx;
     v.dy = g->d.dy;
     v.dx_half_col = g->d.colsweep_uses_dx ? g->d.dx : g->d.dy;

@@ -72,6 +72,7 @@ struct SchemeOpts {
     int limiter;      // 0 minmod (reference), 1 van Leer
     int wave_speeds;  // 0 Garrick-Owkes-Regele (reference), 1 PVRS (Allaire)
     double z_lo, z_hi;
+    int godunov;      // 1: first-order Godunov faces and upwind z update instead of MUSCL-Hancock
 };
 
 CFD_HD double smin(double a, double b) { return (b < a) ? b : a; }
@@ -561,6 +562,25 @@ CFD_HD double zupdate(double r, double z, double usL, double usR, double zsL, do
 {
     double newz = z - r * (usR * zsR - usL * zsL - z * (usR - usL));
     return smax(smin(z_hi, newz), z_lo);
+}
+
+// First-order upwind z update (A/headers/zupdate.hpp:44-50), used with the Godunov flux.
+CFD_HD double zupdate_upwind(double r, double zL, double z, double zR, double usL, double usR, double z_lo,
+                             double z_hi)
+{
+    double newz = z - r * ((zR - z) * smin(0.0, usR) + (z - zL) * smax(0.0, usL));
+    return smax(smin(z_hi, newz), z_lo);
+}
+
+// Godunov face (E/headers/flux_solver_godunov.hpp:30-36, A/headers/flux_solver_godunov.hpp:30-39):
+// HLLC of the two cells' own conserved states; z* as in hllc.
+template <int M>
+CFD_HD void godunov_face(const Material& m, int wave_speeds, const double* Ua, const double* Ub, FaceOut<M>& out)
+{
+    Side<M> L, R;
+    side_from_raw<M>(m, Ua, L);
+    side_from_raw<M>(m, Ub, R);
+    hllc<M>(m, wave_speeds, L, R, out);
 }
 
 // Wave speed |u|+c, |v|+c of a final (unrotated) state: compute_dt's per-cell work

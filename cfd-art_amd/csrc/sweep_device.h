@@ -244,6 +244,91 @@ __device__ __forceinline__ void x_tile(const GridView& g, double dt, int tx, int
     }
 }
 
+// Godunov update of one cell: conservative update, then for Allaire the upwind z update and
+// the snapping physical-state test (A/sourcecode/allaire_diffuse_rowsweep.cpp:36-50 with
+// zupdate_upwind).
+template <int M>
+__device__ __forceinline__ void update_cell_godunov(const GridView& g, double r, const double* U, double zL,
+                                                    double zR, const double* FL, const double* FR, double usL,
+                                                    double usR, double* Un, bool& bad, bool checked)
+{
+    constexpr int NC = Model<M>::NC;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) Un[k] = U[k] + r * (FL[k] - FR[k]);
+    bad = false;
+    if (M == ALLAIRE) {
+        Un[5] = zupdate_upwind(r, zL, U[5], zR, usL, usR, g.opt.z_lo, g.opt.z_hi);
+        snap(Un);
+        bad = checked && !allaire_physical_screened(Un);
+    }
+}
+
+// One x-sweep tile of the first-order Godunov scheme (same tile geometry as x_tile): the face
+// left of each lane's cell is the Riemann problem of the left neighbour's and the cell's stored
+// states.
+template <int M, bool LAST>
+__device__ __forceinline__ void x_tile_godunov(const GridView& g, double dt, int tx, int row0, int nrows, Tally& tl)
+{
+    constexpr int NC = Model<M>::NC;
+    constexpr int MN = Model<M>::MN;
+    const int lane = threadIdx.x & 63;
+    const int x = tx * XTILE - 2 + lane;
+    const bool store = lane >= 2 && lane < 62 && x < g.nx;
+    const double r = dt / g.dx;
+    const int xs = mirror(x, g.nx);
+    const bool neg = (x < 0 && g.bc[0]) || (x >= g.nx && g.bc[1]);
+    const int yend = min(row0 + nrows, g.ny);
+
+    double Unext[NC];
+    if (row0 < yend) {
+        const double* base = g.cur + (long long)row0 * g.pitch + xs;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) Unext[k] = base[k * g.plane];
+    }
+    for (int y = row0; y < yend; ++y) {
+        double U[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) U[k] = Unext[k];
+        if (y + 1 < yend) {
+            const double* base = g.cur + (long long)(y + 1) * g.pitch + xs;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) Unext[k] = base[k * g.plane];
+        }
+        if (neg) U[MN] = -U[MN];
+
+        double Ua[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) Ua[k] = lane_from_left_or_self(U[k]);
+        FaceOut<M> fo;
+        godunov_face<M>(g.mat, g.opt.wave_speeds, Ua, U, fo);
+        double Fr[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) Fr[k] = lane_from_right(fo.F[k]);
+        const double usr = lane_from_right(fo.ustar);
+        const double zL = M == ALLAIRE ? Ua[NC - 1] : 0.0;
+        const double zR = M == ALLAIRE ? lane_from_right_or_self(U[NC - 1]) : 0.0;
+
+        double Un[NC];
+        bool bad;
+        update_cell_godunov<M>(g, r, U, zL, zR, fo.F, Fr, fo.ustar, usr, Un, bad, store);
+        bool rep = false;
+        if (M == EULER && store) rep = euler_repair(g.mat, U, Un);
+        if (store) {
+            double* out = g.nxt + (long long)y * g.pitch + x;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) __builtin_nontemporal_store(Un[k], out + k * g.plane);
+            tl.repair += rep;
+            tl.bad += bad;
+            if (LAST) {
+                double su, sv;
+                wave_speeds(g.mat, Un, su, sv, (Model<M>*)nullptr);
+                tl.su = smax(tl.su, su);
+                tl.sv = smax(tl.sv, sv);
+            }
+        }
+    }
+}
+
 // Identity copy of a tile (a grid whose time loop has ended keeps its state through the swap).
 template <int M>
 __device__ __forceinline__ void copy_rows(const GridView& g, int x, bool valid, int y0, int y1)
@@ -357,6 +442,66 @@ __device__ __forceinline__ void y_tile(const GridView& g, double dt, int bx, int
         P0 = Pp;
         a.fR = cs.fR;
         a.R = cs.R;
+        Fm = fo;
+    }
+}
+
+// One y-sweep tile of the first-order Godunov scheme: each lane marches rows [y0, y1) of its
+// column with a two-row window of stored states (faces y0 .. y1 need rows y0-1 .. y1).
+template <int M, bool LAST>
+__device__ __forceinline__ void y_tile_godunov(const GridView& g, double dt, int bx, int y0, int y1, Tally& tl)
+{
+    constexpr int NC = Model<M>::NC;
+    constexpr int MX = Model<M>::MN, MY = Model<M>::MT;
+    const int x = bx * YBLOCK + threadIdx.x;
+    const bool valid = x < g.nx;
+    const double r = dt / g.dy;
+    const int xs = valid ? x : g.nx - 1;
+
+    double Um[NC], U0[NC], Up[NC], Unext[NC];
+    load_rot<M>(g, y0 - 1, xs, Um);
+    load_rot<M>(g, y0, xs, U0);
+    FaceOut<M> Fm;
+    godunov_face<M>(g.mat, g.opt.wave_speeds, Um, U0, Fm);  // face y0
+    load_rot<M>(g, y0 + 1, xs, Unext);
+    for (int c = y0 + 1; c <= y1; ++c) {
+        // window: Um = row c-2, U0 = row c-1, Up = row c
+#pragma unroll
+        for (int k = 0; k < NC; ++k) Up[k] = Unext[k];
+        if (c + 1 <= y1) load_rot<M>(g, c + 1, xs, Unext);
+        FaceOut<M> fo;
+        godunov_face<M>(g.mat, g.opt.wave_speeds, U0, Up, fo);  // face c
+        // update row c-1 from faces c-1 (Fm) and c (fo)
+        double Un[NC];
+        bool bad;
+        update_cell_godunov<M>(g, r, U0, Um[NC - 1], Up[NC - 1], Fm.F, fo.F, Fm.ustar, fo.ustar, Un, bad, valid);
+        double t = Un[MX];
+        Un[MX] = -Un[MY];
+        Un[MY] = t;
+        bool rep = false;
+        if (M == EULER && valid) {
+            double old[NC];
+#pragma unroll
+            for (int k = 0; k < NC; ++k) old[k] = U0[k];
+            old[MX] = -U0[MY];
+            old[MY] = U0[MX];
+            rep = euler_repair(g.mat, old, Un);
+        }
+        if (valid) {
+            double* out = g.nxt + (long long)(c - 1) * g.pitch + x;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) __builtin_nontemporal_store(Un[k], out + k * g.plane);
+            tl.repair += rep;
+            tl.bad += bad;
+            if (LAST) {
+                double su, sv;
+                wave_speeds(g.mat, Un, su, sv, (Model<M>*)nullptr);
+                tl.su = smax(tl.su, su);
+                tl.sv = smax(tl.sv, sv);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NC; ++k) { Um[k] = U0[k]; U0[k] = Up[k]; }
         Fm = fo;
     }
 }

@@ -18,8 +18,8 @@ namespace cfd {
 
 // The Allaire x sweep runs at 3 wavefronts per SIMD (at most 168 VGPRs; a few registers spill),
 // which measured faster than the 2 its natural 176-180 VGPRs allow.
-template <int M, bool LAST>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M == ALLAIRE ? 3 : 1))) void sweep_x_kernel(GridView g, double dt_arg, double dt_factor, int use_ctrl,
+template <int M, bool LAST, bool GOD>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M == ALLAIRE && !GOD ? 3 : 1))) void sweep_x_kernel(GridView g, double dt_arg, double dt_factor, int use_ctrl,
                                                      int xrows)
 {
     // tiles in row-major order: neighbours along x (which share halo columns) are consecutive
@@ -35,11 +35,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M == ALLAIR
         return;
     }
     Tally tl;
-    x_tile<M, LAST>(g, dt, tx, row0, xrows, tl);
+    if (GOD) x_tile_godunov<M, LAST>(g, dt, tx, row0, xrows, tl);
+    else x_tile<M, LAST>(g, dt, tx, row0, xrows, tl);
     tl.flush(g.ctrl, LAST, blockIdx.x + blockIdx.y * gridDim.x);
 }
 
-template <int M, bool LAST>
+template <int M, bool LAST, bool GOD>
 __global__ __launch_bounds__(YBLOCK) void sweep_y_kernel(GridView g, double dt_arg, double dt_factor,
                                                         int use_ctrl, int H)
 {
@@ -56,7 +57,8 @@ __global__ __launch_bounds__(YBLOCK) void sweep_y_kernel(GridView g, double dt_a
         return;
     }
     Tally tl;
-    y_tile<M, LAST>(g, dt, bx, y0, y1, tl);
+    if (GOD) y_tile_godunov<M, LAST>(g, dt, bx, y0, y1, tl);
+    else y_tile<M, LAST>(g, dt, bx, y0, y1, tl);
     tl.flush(g.ctrl, LAST, blockIdx.x + blockIdx.y * gridDim.x);
 }
 
@@ -151,34 +153,38 @@ hipError_t launch_set_time(Ctrl* c, double t, int n, hipStream_t st)
     return hipGetLastError();
 }
 
-template <int M, bool LAST>
+template <int M, bool LAST, bool GOD>
 static hipError_t launch_x(const GridView& g, double dt_arg, double f, int use_ctrl, int xrows, hipStream_t st)
 {
     dim3 grid((g.nx + XTILE - 1) / XTILE, (g.ny + 4 * xrows - 1) / (4 * xrows));
-    hipLaunchKernelGGL((sweep_x_kernel<M, LAST>), grid, dim3(256), 0, st, g, dt_arg, f, use_ctrl, xrows);
+    hipLaunchKernelGGL((sweep_x_kernel<M, LAST, GOD>), grid, dim3(256), 0, st, g, dt_arg, f, use_ctrl, xrows);
     return hipGetLastError();
 }
-template <int M, bool LAST>
+template <int M, bool LAST, bool GOD>
 static hipError_t launch_y(const GridView& g, double dt_arg, double f, int use_ctrl, int H, hipStream_t st)
 {
     dim3 grid((g.nx + YBLOCK - 1) / YBLOCK, (g.ny + H - 1) / H);
-    hipLaunchKernelGGL((sweep_y_kernel<M, LAST>), grid, dim3(YBLOCK), 0, st, g, dt_arg, f, use_ctrl, H);
+    hipLaunchKernelGGL((sweep_y_kernel<M, LAST, GOD>), grid, dim3(YBLOCK), 0, st, g, dt_arg, f, use_ctrl, H);
     return hipGetLastError();
+}
+template <int M, bool GOD>
+static hipError_t launch_model(const GridView& g, int axis, bool last, double dt_arg, double dt_factor, int use_ctrl,
+                               int yH, int xrows, hipStream_t st)
+{
+    if (axis == 0) return last ? launch_x<M, true, GOD>(g, dt_arg, dt_factor, use_ctrl, xrows, st)
+                               : launch_x<M, false, GOD>(g, dt_arg, dt_factor, use_ctrl, xrows, st);
+    return last ? launch_y<M, true, GOD>(g, dt_arg, dt_factor, use_ctrl, yH, st)
+                : launch_y<M, false, GOD>(g, dt_arg, dt_factor, use_ctrl, yH, st);
 }
 
 hipError_t launch_sweep(const GridView& g, int model, int axis, bool last, double dt_arg,
                         double dt_factor, int use_ctrl, int yH, int xrows, hipStream_t st)
 {
-    if (model == EULER) {
-        if (axis == 0) return last ? launch_x<EULER, true>(g, dt_arg, dt_factor, use_ctrl, xrows, st)
-                                   : launch_x<EULER, false>(g, dt_arg, dt_factor, use_ctrl, xrows, st);
-        return last ? launch_y<EULER, true>(g, dt_arg, dt_factor, use_ctrl, yH, st)
-                    : launch_y<EULER, false>(g, dt_arg, dt_factor, use_ctrl, yH, st);
-    }
-    if (axis == 0) return last ? launch_x<ALLAIRE, true>(g, dt_arg, dt_factor, use_ctrl, xrows, st)
-                               : launch_x<ALLAIRE, false>(g, dt_arg, dt_factor, use_ctrl, xrows, st);
-    return last ? launch_y<ALLAIRE, true>(g, dt_arg, dt_factor, use_ctrl, yH, st)
-                : launch_y<ALLAIRE, false>(g, dt_arg, dt_factor, use_ctrl, yH, st);
+    if (model == EULER)
+        return g.opt.godunov ? launch_model<EULER, true>(g, axis, last, dt_arg, dt_factor, use_ctrl, yH, xrows, st)
+                             : launch_model<EULER, false>(g, axis, last, dt_arg, dt_factor, use_ctrl, yH, xrows, st);
+    return g.opt.godunov ? launch_model<ALLAIRE, true>(g, axis, last, dt_arg, dt_factor, use_ctrl, yH, xrows, st)
+                         : launch_model<ALLAIRE, false>(g, axis, last, dt_arg, dt_factor, use_ctrl, yH, xrows, st);
 }
 
 int reduce_blocks(const GridView& g)

@@ -161,7 +161,8 @@ std::string GpuProblem::set_ICs(const Settings& sf)
     sf_ = sf;
     outname_ = sf.basename;
     cfl_ = sf.CFL;
-    if (sf.flux_solver != "MUSCL") return "flux_solver must be MUSCL (the Godunov scheme is not built)";
+    // the flux solver and its ghost width (euler_io.cpp:446-457, allaire_diffuse_io.cpp:412-423)
+    if (sf.flux_solver != "MUSCL" && sf.flux_solver != "Godunov") return "flux_solver must be MUSCL or Godunov";
     if (sf.riemann_solver != "HLLC") return "riemann_solver must be HLLC";
     if (sf.splitting_scheme == "naive_splitting") split_ = CFDART_NAIVE;
     else if (sf.splitting_scheme == "strang_splitting") split_ = CFDART_STRANG;
@@ -172,7 +173,8 @@ std::string GpuProblem::set_ICs(const Settings& sf)
     d = cfdart_desc{};
     d.nx = sf.Nx;
     d.ny = sf.Ny;
-    d.ng = 2;
+    d.flux_solver = sf.flux_solver == "Godunov" ? CFDART_GODUNOV : CFDART_MUSCL_HANCOCK;
+    d.ng = sf.flux_solver == "Godunov" ? 1 : 2;
     d.limiter = sf.limiter == "vanleer" ? CFDART_VANLEER : CFDART_MINMOD;
     d.wave_speeds = sf.wave_speeds == "pvrs" ? CFDART_PVRS : CFDART_GARRICK;
     d.colsweep_uses_dx = sf.colsweep_uses_dx;
@@ -183,7 +185,7 @@ std::string GpuProblem::set_ICs(const Settings& sf)
         d.x0 = x0; d.y0 = y0; d.dx = lx / d.nx; d.dy = ly / d.ny; T_ = T;
     };
     const std::string& tc = sf.test_case;
-    host_.assign((size_t)(d.nx + 4) * (d.ny + 4) * (sf.problem == "allaire" ? 6 : 4), 0.0);
+    host_.assign((size_t)(d.nx + 2 * d.ng) * (d.ny + 2 * d.ng) * (sf.problem == "allaire" ? 6 : 4), 0.0);
 
     if (sf.problem == "euler") {
         // set_parameters (euler_io.cpp:13-234)

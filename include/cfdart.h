@@ -40,13 +40,15 @@ enum { CFDART_TRANSMISSIVE = 0, CFDART_REFLECTIVE = 1 };
 enum { CFDART_MINMOD = 0, CFDART_VANLEER = 1 };
 enum { CFDART_GARRICK = 0, CFDART_PVRS = 1 };
 enum { CFDART_NAIVE = 0, CFDART_STRANG = 1, CFDART_ALTERNATING = 2 };
+enum { CFDART_MUSCL_HANCOCK = 0, CFDART_GODUNOV = 1 };
 
 /* Grid and scheme description; mirrors the reference's sim_info + SGparams/binarySGparams +
  * the flux-solver choices (euler_equations/euler_simulation_code/headers/sim_info.hpp:16-63,
  * headers/stiffened_gas_eos.hpp:20-50, sourcecode/euler_io.cpp:13-234,436-491). */
 typedef struct cfdart_desc {
     int model;            /* CFDART_EULER or CFDART_ALLAIRE */
-    int nx, ny, ng;       /* interior cells (>= 2 each) and ghost width (must be 2: MUSCL) */
+    int nx, ny, ng;       /* interior cells (>= 2 each) and ghost width of the host format:
+                             2 for MUSCL-Hancock, 1 (the reference's) or 2 for Godunov */
     double x0, y0, dx, dy;
     double gamma1, pinf1; /* Euler material / Allaire fluid 1 */
     double gamma2, pinf2; /* Allaire fluid 2 (ignored for Euler) */
@@ -55,6 +57,9 @@ typedef struct cfdart_desc {
     int wave_speeds;      /* CFDART_GARRICK (reference) or CFDART_PVRS (Allaire only) */
     int colsweep_uses_dx; /* 1 = reference: the column sweep's half-step uses dx */
     double z_lo, z_hi;    /* Allaire volume-fraction clamp (reference 0 and 1) */
+    int flux_solver;      /* CFDART_MUSCL_HANCOCK or CFDART_GODUNOV: first order, the Riemann
+                             problem of the two cells' own states and, for Allaire, the upwind
+                             z update (flux_solver_godunov.hpp:18-41, A/headers/zupdate.hpp:40-55) */
 } cfdart_desc;
 
 typedef struct cfdart_grid cfdart_grid;

@@ -12,7 +12,8 @@
  *   MUSCL-Hancock face  E/headers/flux_solver_MUSCLhancock.hpp:128-227 ; A/headers/flux_solver_MUSCLhancock.hpp:272-376
  *   HLLC                E/headers/riemann_solver_HLLC.hpp:50-145 ; A/headers/riemann_solver_HLLC.hpp:49-150
  *                       (PVRS wave speeds: the commented block A/headers/riemann_solver_HLLC.hpp:84-107)
- *   z update            A/headers/zupdate.hpp:58-73
+ *   z update            A/headers/zupdate.hpp:58-73 (second order), :40-55 (upwind, with Godunov)
+ *   Godunov face        E/headers/flux_solver_godunov.hpp:18-41 ; A/headers/flux_solver_godunov.hpp:17-41
  *   sweeps              E/sourcecode/euler_rowsweep.cpp:10-95, euler_colsweep.cpp:10-114,
  *                       A/sourcecode/allaire_diffuse_rowsweep.cpp:10-52, allaire_diffuse_colsweep.cpp:9-73
  *   ghost fill / swap   E/sourcecode/euler_ppsweep.cpp:7-123 ; A/sourcecode/allaire_diffuse_ppsweep.cpp:7-123
@@ -31,7 +32,8 @@
  * round trips; Jacobian vs finite differences; limiter tables; HLLC consistency and
  * contact preservation; free-stream preservation; x/y transposition invariance; conservation
  * budgets; Toro's exact Riemann solutions (L1 convergence); ghost-fill index tables by
- * brute force; dt by brute force. Parity unpinned: the exact Eigen evaluation order of the
+ * brute force; dt by brute force; the Godunov scheme by first-order convergence to the exact
+ * Riemann solution and the upwind z update by its closed form on a step. Parity unpinned: the exact Eigen evaluation order of the
  * half-step product (reassociation-level only; no reference binary can be built, Eigen is
  * absent) and the reference's racy multithreaded max in compute_dt.
  */
@@ -433,6 +435,27 @@ double orc_zupdate(const orc_desc* d, double dx, double dt, double zL, double z,
     return std_max(std_min(d->z_hi, newz), d->z_lo);
 }
 
+/* A/headers/zupdate.hpp:44-50 (upwind; paired with the Godunov flux solver,
+ * A/sourcecode/allaire_diffuse_io.cpp:446-450), clamp bounds parameterised */
+double orc_zupdate_upwind(const orc_desc* d, double dx, double dt, double zL, double z, double zR,
+                          double usL, double usR)
+{
+    double newz = z - (dt / dx) * ((zR - z) * std_min(0.0, usR) + (z - zL) * std_max(0.0, usL));
+    return std_max(std_min(d->z_hi, newz), d->z_lo);
+}
+
+/* Godunov face (E/headers/flux_solver_godunov.hpp:30-36, A/headers/flux_solver_godunov.hpp:30-39):
+ * the Riemann problem of the two cells' stored states, z* as the MUSCL face does. */
+static void godunov_face(const orc_desc* d, const double* s0, const double* s1, double* F, double* ustar,
+                         double* zstar)
+{
+    orc_hllc(d, s0, s1, F, ustar);
+    if (d->model == ORC_ALLAIRE) {
+        double sg = copysign(1.0, *ustar);
+        *zstar = ((1.0 + sg) / 2.0) * s0[5] + ((1.0 - sg) / 2.0) * s1[5];
+    }
+}
+
 /* ------------------------------------------------------------------ ghost fill */
 
 /* E/sourcecode/euler_ppsweep.cpp:7-113 ; A/sourcecode/allaire_diffuse_ppsweep.cpp:7-113.
@@ -499,10 +522,17 @@ static void sweep_line(const orc_desc* d, const double* line, int len, double dt
                        double* out, long long* counters)
 {
     int n = orc_ncomp(d->model), g = d->ng;
-    for (int j = g; j < len + g + 1; ++j)
-        orc_face(d, line + (size_t)(j - 2) * n, line + (size_t)(j - 1) * n, line + (size_t)j * n,
-                 line + (size_t)(j + 1) * n, dt, dx_half, flux + (size_t)j * n, ustar + j,
-                 zstar + j, counters);
+    int god = d->flux_solver == ORC_GODUNOV;
+    /* E/sourcecode/euler_rowsweep.cpp:23-32: faces j in [g, len+g], stencil j-stclsize .. j+stclsize-1 */
+    for (int j = g; j < len + g + 1; ++j) {
+        if (god)
+            godunov_face(d, line + (size_t)(j - 1) * n, line + (size_t)j * n, flux + (size_t)j * n, ustar + j,
+                         zstar + j);
+        else
+            orc_face(d, line + (size_t)(j - 2) * n, line + (size_t)(j - 1) * n, line + (size_t)j * n,
+                     line + (size_t)(j + 1) * n, dt, dx_half, flux + (size_t)j * n, ustar + j,
+                     zstar + j, counters);
+    }
     double r = dt / dx_update;
     for (int j = g; j < len + g; ++j)
         for (int k = 0; k < n; ++k)
@@ -512,9 +542,13 @@ static void sweep_line(const orc_desc* d, const double* line, int len, double dt
         /* A/sourcecode/allaire_diffuse_rowsweep.cpp:44-50 */
         for (int j = g; j < len + g; ++j) {
             double* o = out + (size_t)(j - g) * n;
-            o[5] = orc_zupdate(d, dx_update, dt, line[(size_t)(j - 1) * n + 5], line[(size_t)j * n + 5],
-                               line[(size_t)(j + 1) * n + 5], ustar[j], ustar[j + 1], zstar[j],
-                               zstar[j + 1]);
+            if (god)
+                o[5] = orc_zupdate_upwind(d, dx_update, dt, line[(size_t)(j - 1) * n + 5], line[(size_t)j * n + 5],
+                                          line[(size_t)(j + 1) * n + 5], ustar[j], ustar[j + 1]);
+            else
+                o[5] = orc_zupdate(d, dx_update, dt, line[(size_t)(j - 1) * n + 5], line[(size_t)j * n + 5],
+                                   line[(size_t)(j + 1) * n + 5], ustar[j], ustar[j + 1], zstar[j],
+                                   zstar[j + 1]);
             if (!orc_is_physical_snap(d, o) && counters) {
 #pragma omp atomic
                 counters[ORC_CNT_UNPHYSICAL] += 1;

@@ -28,6 +28,7 @@ enum { ORC_TRANSMISSIVE = 0, ORC_REFLECTIVE = 1 };
 enum { ORC_MINMOD = 0, ORC_VANLEER = 1 };
 enum { ORC_GARRICK = 0, ORC_PVRS = 1 };
 enum { ORC_NAIVE = 0, ORC_STRANG = 1, ORC_ALTERNATING = 2 };
+enum { ORC_MUSCL_HANCOCK = 0, ORC_GODUNOV = 1 };
 
 /* Counters of the reference's diagnostic prints / aborts. */
 enum {
@@ -40,7 +41,7 @@ enum {
 
 typedef struct orc_desc {
     int model;            /* ORC_EULER (4 comps) or ORC_ALLAIRE (6 comps) */
-    int nx, ny, ng;       /* interior cells and ghost width (2 for MUSCL) */
+    int nx, ny, ng;       /* interior cells and ghost width (2 for MUSCL, 1 for Godunov) */
     double x0, y0, dx, dy;
     double gamma1, pinf1; /* Euler material, or Allaire fluid 1 */
     double gamma2, pinf2; /* Allaire fluid 2 */
@@ -49,6 +50,7 @@ typedef struct orc_desc {
     int wave_speeds;      /* ORC_GARRICK (reference) or ORC_PVRS (Allaire only) */
     int colsweep_uses_dx; /* 1 = reference: column half-step uses dx */
     double z_lo, z_hi;    /* Allaire volume-fraction clamp (reference 0, 1) */
+    int flux_solver;      /* ORC_MUSCL_HANCOCK or ORC_GODUNOV */
 } orc_desc;
 
 int  orc_ncomp(int model);
@@ -78,6 +80,9 @@ int  orc_is_physical(const orc_desc* d, const double* U);          /* const over
 int  orc_is_physical_snap(const orc_desc* d, double* U);           /* Allaire snapping overload */
 void orc_flux_phys(const orc_desc* d, const double* U, double* F);
 void orc_hllc(const orc_desc* d, const double* UL, const double* UR, double* F, double* ustar);
+/* First-order upwind z update (A/headers/zupdate.hpp:44-50), clamp bounds parameterised. */
+double orc_zupdate_upwind(const orc_desc* d, double dx, double dt, double zL, double z, double zR,
+                          double usL, double usR);
 void orc_face(const orc_desc* d, const double* s0, const double* s1, const double* s2,
               const double* s3, double dt, double dx, double* F, double* ustar,
               double* zstar, long long* counters);

@@ -17,6 +17,7 @@ class OrcDesc(C.Structure):
         ("gamma1", C.c_double), ("pinf1", C.c_double), ("gamma2", C.c_double), ("pinf2", C.c_double),
         ("bc", C.c_int * 4), ("limiter", C.c_int), ("wave_speeds", C.c_int),
         ("colsweep_uses_dx", C.c_int), ("z_lo", C.c_double), ("z_hi", C.c_double),
+        ("flux_solver", C.c_int),
     ]
 
 

@@ -24,25 +24,36 @@ CASES = {
                                    steps=10, cfl=0.5),
     "allaire_48x32_garrick_naive": dict(kind="allaire", nx=48, ny=32, ws=cfdart.GARRICK, splitting=cfdart.NAIVE,
                                         steps=10, cfl=0.5),
+    "euler_48x32_godunov_naive": dict(kind="euler", nx=48, ny=32, limiter=cfdart.MINMOD, bc=(1, 0, 0, 1),
+                                      splitting=cfdart.NAIVE, steps=10, cfl=0.8, godunov=True),
+    "allaire_48x32_godunov_alt": dict(kind="allaire", nx=48, ny=32, ws=cfdart.PVRS, splitting=cfdart.ALTERNATING,
+                                      steps=10, cfl=0.5, godunov=True),
 }
 
 
 def build_case(c):
     if c["kind"] == "euler":
-        d, h = cfdart.configs.euler_discs(c["nx"], 5, (0.05, 0.2), c["bc"], seed=21, limiter=c["limiter"], ny=c["ny"])
+        d, p = cfdart.configs.euler_discs_params(c["nx"], 5, (0.05, 0.2), c["bc"], seed=21, limiter=c["limiter"],
+                                                 ny=c["ny"])
     else:
-        d, h = cfdart.configs.allaire_bubbles(c["nx"], 3, seed=21, ny=c["ny"], wave_speeds=c["ws"])
+        d, p = cfdart.configs.allaire_bubbles_params(c["nx"], 3, seed=21, ny=c["ny"], wave_speeds=c["ws"])
+    if c.get("godunov"):
+        d.flux_solver, d.ng = cfdart.GODUNOV, 1
+    h = cfdart.random_discs(d, p) if c["kind"] == "euler" else cfdart.shocked_bubbles(d, p)
     return d, h
 
 
 def desc_array(d):
     return np.array([d.model, d.nx, d.ny, d.ng, d.x0, d.y0, d.dx, d.dy, d.gamma1, d.pinf1, d.gamma2, d.pinf2,
                      d.bc[0], d.bc[1], d.bc[2], d.bc[3], d.limiter, d.wave_speeds, d.colsweep_uses_dx,
-                     d.z_lo, d.z_hi], dtype=np.float64)
+                     d.z_lo, d.z_hi, d.flux_solver], dtype=np.float64)
 
 
 def main():
+    force = "--force" in sys.argv[1:]
     for name, c in CASES.items():
+        if not force and os.path.exists(os.path.join(HERE, name + ".npz")):
+            continue
         d, h = build_case(c)
         s = oracle.Sim(d, h)
         k, hist = s.run(c["splitting"], c["steps"], cfl=c["cfl"])
