@@ -75,6 +75,7 @@ def lib():
         L.cfdart_compute_dt.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, dp, dp]
         L.cfdart_step.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double]
         L.cfdart_run.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, dp, P(C.c_int)]
+        L.cfdart_run_mass.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, dp, P(C.c_int), dp]
         L.cfdart_set_time.argtypes = [C.c_void_p, C.c_double, C.c_int]
         L.cfdart_stats.argtypes = [C.c_void_p, P(C.c_longlong)]
         L.cfdart_sync.argtypes = [C.c_void_p]
@@ -92,6 +93,7 @@ def lib():
         L.cfdart_init_shocked_bubbles.argtypes = [C.c_void_p, P(BubbleParams)]
         L.cfdart_checksum.argtypes = [C.c_void_p, P(C.c_ulonglong)]
         L.cfdart_frame.argtypes = [C.c_void_p, C.c_void_p]
+        L.cfdart_schlieren.argtypes = [C.c_void_p, dp, dp]
         # batch (include/cfdart_batch.h)
         L.cfdart_batch_create.argtypes = [C.c_int, C.c_size_t, P(C.c_void_p)]
         L.cfdart_batch_destroy.argtypes = [C.c_void_p]
@@ -255,6 +257,22 @@ class Grid:
         v = C.c_ulonglong()
         _check(lib().cfdart_checksum(self.h, C.byref(v)))
         return v.value
+
+    def schlieren(self):
+        """(image [ny, nx], max |grad rho|) of the current state, computed on the device."""
+        img = np.empty((self.desc.ny, self.desc.nx), dtype=np.float64)
+        m = C.c_double()
+        _check(lib().cfdart_schlieren(self.h, _dptr(img), C.byref(m)))
+        return img, m.value
+
+    def run_mass(self, splitting, max_steps, T=math.inf, cfl=0.8):
+        """run() that also returns the mass history: array [steps taken, 3] of (t, mass1, mass2)."""
+        t = C.c_double()
+        n = C.c_int()
+        hist = np.zeros((max(max_steps, 1), 3))
+        n0 = self.run(splitting, 0, T, cfl)[1] if max_steps else 0
+        _check(lib().cfdart_run_mass(self.h, splitting, max_steps, T, cfl, C.byref(t), C.byref(n), _dptr(hist)))
+        return t.value, n.value, hist[: n.value - n0]
 
     def stats(self):
         c = (C.c_longlong * 4)()

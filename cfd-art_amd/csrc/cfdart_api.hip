@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "cfdart_host.h"
+#include "frames.h"
 #include "ic_device.h"
 #include "sweep_kernels.h"
 
@@ -306,6 +307,20 @@ int cfdart_checksum(cfdart_grid* g, unsigned long long* out)
     return 0;
 }
 
+int cfdart_schlieren(cfdart_grid* g, double* host_dst, double* max_grad)
+{
+    if (!g || !host_dst) return fail(CFDART_E_ARG, "null argument");
+    // the image is built in the grid's other buffer, which holds nothing between sweeps
+    double* img = g->buf[1 - g->cur];
+    HIPCHK(cfd::launch_schlieren(view(g), g->d.model, img, g->d_sum, g->st));
+    HIPCHK(hipMemcpyAsync(host_dst, img, sizeof(double) * (size_t)g->d.nx * g->d.ny, hipMemcpyDeviceToHost, g->st));
+    unsigned long long m = 0;
+    HIPCHK(hipMemcpyAsync(&m, g->d_sum, sizeof(m), hipMemcpyDeviceToHost, g->st));
+    HIPCHK(hipStreamSynchronize(g->st));
+    if (max_grad) std::memcpy(max_grad, &m, sizeof(double));
+    return 0;
+}
+
 int cfdart_frame(cfdart_grid* g, double* host_dst)
 {
     if (!g || !host_dst) return fail(CFDART_E_ARG, "null argument");
@@ -417,7 +432,8 @@ int cfdart_set_time(cfdart_grid* g, double t, int n)
     return 0;
 }
 
-int cfdart_run(cfdart_grid* g, int splitting, int max_steps, double T, double cfl, double* t_out, int* n_out)
+int cfdart_run_mass(cfdart_grid* g, int splitting, int max_steps, double T, double cfl, double* t_out, int* n_out,
+                    double* mass_hist)
 {
     if (!g || splitting < 0 || splitting > 2 || max_steps < 0) return fail(CFDART_E_ARG, "bad argument");
     if (!(cfl > 0.0)) return fail(CFDART_E_ARG, "cfl must be positive");
@@ -435,16 +451,44 @@ int cfdart_run(cfdart_grid* g, int splitting, int max_steps, double T, double cf
         HIPCHK(hipStreamSynchronize(g->st));
         n0 = c.n;
     }
+    double* d_hist = nullptr;
+    if (mass_hist && max_steps > 0) {
+        HIPCHK(hipMalloc(&d_hist, sizeof(double) * 3 * (size_t)max_steps));
+        HIPCHK(hipMemsetAsync(d_hist, 0, sizeof(double) * 3 * (size_t)max_steps, g->st));
+    }
+    auto release = [&]() {
+        if (d_hist) (void)hipFree(d_hist);
+    };
     for (int k = 0; k < max_steps; ++k) {
+        if (d_hist) {
+            e = cfd::launch_mass_record(view(g), g->d.model, g->partials, d_hist + 3 * (size_t)k, g->st);
+            if (e != hipSuccess) {
+                release();
+                return fail(CFDART_E_HIP, hipGetErrorString(e));
+            }
+        }
         int axes[3];
         double f[3];
         int ns = step_plan(splitting, n0 + k, axes, f);
         for (int s = 0; s < ns; ++s) {
             int rc = sweep_once(g, axes[s], s == ns - 1, 0.0, f[s], 1);
-            if (rc) return rc;
+            if (rc) {
+                release();
+                return rc;
+            }
         }
         e = cfd::launch_finalize(g->ctrl, g->d.dx, g->d.dy, g->st);
-        if (e != hipSuccess) return fail(CFDART_E_HIP, hipGetErrorString(e));
+        if (e != hipSuccess) {
+            release();
+            return fail(CFDART_E_HIP, hipGetErrorString(e));
+        }
+    }
+    if (d_hist) {
+        hipError_t ce = hipMemcpyAsync(mass_hist, d_hist, sizeof(double) * 3 * (size_t)max_steps,
+                                       hipMemcpyDeviceToHost, g->st);
+        if (ce == hipSuccess) ce = hipStreamSynchronize(g->st);
+        release();
+        if (ce != hipSuccess) return fail(CFDART_E_HIP, hipGetErrorString(ce));
     }
     if (t_out || n_out) {
         cfd::Ctrl c;
@@ -455,6 +499,11 @@ int cfdart_run(cfdart_grid* g, int splitting, int max_steps, double T, double cf
         if (c.error) return fail(CFDART_E_UNPHYS, "an Allaire state failed the physical-state test");
     }
     return 0;
+}
+
+int cfdart_run(cfdart_grid* g, int splitting, int max_steps, double T, double cfl, double* t_out, int* n_out)
+{
+    return cfdart_run_mass(g, splitting, max_steps, T, cfl, t_out, n_out, nullptr);
 }
 
 int cfdart_stats(cfdart_grid* g, long long* counters)

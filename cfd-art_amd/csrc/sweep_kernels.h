@@ -47,6 +47,9 @@ hipError_t launch_sweep(const GridView& g, int model, int axis, bool last, doubl
                         double dt_factor, int use_ctrl, int yH, int xrows, hipStream_t st);
 hipError_t launch_reduce_dt(const GridView& g, int model, double* partials, hipStream_t st);
 hipError_t launch_init_dt(Ctrl* c, double dx, double dy, hipStream_t st);
+// Mass history entry of the current state (t, mass1, mass2 into hist[0..2]) when the time loop
+// is running: the sums compute_dt records, from the same per-block partials.
+hipError_t launch_mass_record(const GridView& g, int model, double* partials, double* hist, hipStream_t st);
 hipError_t launch_finalize(Ctrl* c, double dx, double dy, hipStream_t st);
 hipError_t launch_set_run(Ctrl* c, double T, double cfl, hipStream_t st);
 hipError_t launch_set_time(Ctrl* c, double t, int n, hipStream_t st);

@@ -104,6 +104,50 @@ __global__ __launch_bounds__(256) void reduce_dt_kernel(GridView g, double* part
     }
 }
 
+// The mass sums of compute_dt (E/sourcecode/euler_timestep.cpp:26,32-33): the same per-block
+// partials as reduce_dt_kernel (same decomposition, same order), reading the density planes only.
+template <int M>
+__global__ __launch_bounds__(256) void mass_kernel(GridView g, double* partials)
+{
+    __shared__ double sm[2][4];
+    const long long n = (long long)g.nx * g.ny;
+    double m1 = 0.0, m2 = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        int y = (int)(i / g.nx), x = (int)(i % g.nx);
+        const double* b = g.cur + (long long)y * g.pitch + x;
+        m1 += b[0];
+        if (M == ALLAIRE) m2 += b[g.plane];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) { m1 += __shfl_xor(m1, o); m2 += __shfl_xor(m2, o); }
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        sm[0][w] = m1;
+        sm[1][w] = m2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = ((sm[0][0] + sm[0][1]) + (sm[0][2] + sm[0][3]));
+        partials[2 * blockIdx.x + 1] = ((sm[1][0] + sm[1][1]) + (sm[1][2] + sm[1][3]));
+    }
+}
+
+// One entry of the mass history, as compute_dt records it at the start of a step that runs
+// (E/sourcecode/euler_timestep.cpp:32-33): t, then masssum * dx * dy per fluid.
+__global__ void mass_record_kernel(const Ctrl* c, const double* partials, int nb, double dx, double dy, double* hist)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0 || !c->active) return;
+    double m1 = 0.0, m2 = 0.0;
+    for (int b = 0; b < nb; ++b) {
+        m1 += partials[2 * b];
+        m2 += partials[2 * b + 1];
+    }
+    hist[0] = c->t;
+    hist[1] = m1 * dx * dy;
+    hist[2] = m2 * dx * dy;
+}
+
 // Start of the time loop: dt for (n, t) from a fresh reduction, and the 'while (t < T)' test.
 __global__ void init_dt_kernel(Ctrl* c, double dx, double dy)
 {
@@ -199,6 +243,17 @@ hipError_t launch_reduce_dt(const GridView& g, int model, double* partials, hipS
     int nb = reduce_blocks(g);
     if (model == EULER) hipLaunchKernelGGL(reduce_dt_kernel<EULER>, dim3(nb), dim3(256), 0, st, g, partials);
     else hipLaunchKernelGGL(reduce_dt_kernel<ALLAIRE>, dim3(nb), dim3(256), 0, st, g, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_mass_record(const GridView& g, int model, double* partials, double* hist, hipStream_t st)
+{
+    const int nb = reduce_blocks(g);
+    if (model == EULER) hipLaunchKernelGGL(mass_kernel<EULER>, dim3(nb), dim3(256), 0, st, g, partials);
+    else hipLaunchKernelGGL(mass_kernel<ALLAIRE>, dim3(nb), dim3(256), 0, st, g, partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(mass_record_kernel, dim3(1), dim3(64), 0, st, g.ctrl, partials, nb, g.dx, g.dy, hist);
     return hipGetLastError();
 }
 

@@ -375,27 +375,15 @@ void GpuProblem::write_state(double t)
 
 void GpuProblem::write_schlieren(double t)
 {
-    // euler_io.cpp:1142-1185: |grad rho| / max |grad rho| by central differences; the ghost
-    // frame here is freshly mirrored from the current state (cfdart_download)
+    // euler_io.cpp:1142-1185: |grad rho| / max |grad rho| by central differences, computed on
+    // the device (cfdart_schlieren); the ghost frame is the mirror of the current state
     const cfdart_desc& d = d_;
-    Field f(d, host_);
-    auto rho = [&](int i, int j) {
-        const double* U = f.at(i, j);
-        return d.model == CFDART_EULER ? U[0] : U[0] + U[1];
-    };
-    std::vector<double> g;
-    g.reserve((size_t)d.nx * d.ny);
-    for (int i = d.ng; i < d.ny + d.ng; ++i)
-        for (int j = d.ng; j < d.nx + d.ng; ++j) {
-            double gx = (rho(i, j + 1) - rho(i, j - 1)) / (2.0 * d.dx);
-            double gy = (rho(i + 1, j) - rho(i - 1, j)) / (2.0 * d.dy);
-            g.push_back(std::sqrt(gx * gx + gy * gy));
-        }
-    double mx = *std::max_element(g.begin(), g.end());
+    std::vector<double> g((size_t)d.nx * d.ny);
+    if (cfdart_schlieren(g_, g.data(), nullptr) != 0) return;
     std::ofstream out(outname_ + "-schlieren-" + std::to_string(t) + ".dat");
     size_t c = 0;
     for (int i = d.ng; i < d.ny + d.ng; ++i) {
-        for (int j = d.ng; j < d.nx + d.ng; ++j) out << ccx(d, j) << " " << ccy(d, i) << " " << g[c++] / mx << "\n";
+        for (int j = d.ng; j < d.nx + d.ng; ++j) out << ccx(d, j) << " " << ccy(d, i) << " " << g[c++] << "\n";
         out << "\n";
     }
 }

@@ -102,6 +102,14 @@ int cfdart_step(cfdart_grid* g, int splitting, int n, double dt);
  * reports the final t and n. Pass T = INFINITY for a fixed number of steps. */
 int cfdart_run(cfdart_grid* g, int splitting, int max_steps, double T, double cfl,
                double* t_out, int* n_out);
+/* As cfdart_run, also recording the mass history of compute_dt (E/sourcecode/euler_timestep.cpp:
+ * 26,32-33, the reference's *-masschange.dat data) on the device: for each step taken,
+ * mass_hist[3k..3k+2] = t at the start of the step and the masses (sum of U0, and of U1 for
+ * Allaire, times dx*dy) of the state it starts from, summed in the same fixed order as
+ * cfdart_compute_dt's mass_out. mass_hist holds 3*max_steps doubles; entries of steps not taken
+ * are 0. Synchronous. */
+int cfdart_run_mass(cfdart_grid* g, int splitting, int max_steps, double T, double cfl,
+                    double* t_out, int* n_out, double* mass_hist);
 /* Sets the loop state used by cfdart_run (defaults after upload: t = 0, n = 0). */
 int cfdart_set_time(cfdart_grid* g, double t, int n);
 /* Counters accumulated since upload (see CFDART_CNT_*), 4 values; synchronous. */
@@ -124,6 +132,13 @@ int cfdart_set_tile(cfdart_grid* g, int y_strip_rows, int x_rows_per_wave);
  * checksums whatever their device layout; used to compare a grid run alone with the same grid
  * in a batch. Synchronous. No reference counterpart. */
 int cfdart_checksum(cfdart_grid* g, unsigned long long* out);
+/* Schlieren image of the current state, computed on the device: per interior cell
+ * |grad rho| / max |grad rho| with central differences, rho = U0 (Euler) or U0 + U1 (Allaire),
+ * as the reference's gnuplot_schlieren (euler_io.cpp:1142-1185, allaire_diffuse_io.cpp:1109-1150).
+ * Cells next to the boundary difference against the boundary condition's mirror of the current
+ * state (the reference reads whatever its ghost cells hold since their last fill). host_dst:
+ * nx*ny doubles, row-major (y outer); max_grad (nullable) receives max |grad rho|. Synchronous. */
+int cfdart_schlieren(cfdart_grid* g, double* host_dst, double* max_grad);
 /* Frame: asynchronous copy of the current interior state to host_dst (pinned memory, ncomp SoA
  * planes of nx*ny doubles) on the grid's stream; complete after cfdart_sync. The state the
  * reference's output (euler_io.cpp:1142-1261) would write, without its ghost frame. */
