@@ -433,7 +433,20 @@ double GpuProblem::compute_dt(int n, double t)
 
 int GpuProblem::advance(int n, double dt) { return cfdart_step(g_, split_, n, dt); }
 
-int GpuProblem::run_steps(int nsteps) { return cfdart_run(g_, split_, nsteps, T_, cfl_, nullptr, nullptr); }
+int GpuProblem::run_steps(int nsteps, double* t, int* n)
+{
+    // the device loop records compute_dt's mass history as it goes (cfdart_run_mass)
+    std::vector<double> hist(3 * (size_t)std::max(nsteps, 1));
+    const int n0 = *n;
+    int rc = cfdart_run_mass(g_, split_, nsteps, INFINITY, sf_.CFL, t, n, hist.data());
+    if (rc) return rc;
+    for (int k = 0; k < *n - n0; ++k) {
+        time_.push_back(hist[3 * (size_t)k]);
+        mass1_.push_back(hist[3 * (size_t)k + 1]);
+        mass2_.push_back(hist[3 * (size_t)k + 2]);
+    }
+    return 0;
+}
 
 // ------------------------------------------------------------------ time loop
 
@@ -455,8 +468,7 @@ int run_simulation(const Settings& sf, std::string* err)
         int chunk = sf.frame_every_steps > 0 ? sf.frame_every_steps : (int)std::min<long long>(left, 1000);
         while (left > 0) {
             int k = (int)std::min<long long>(chunk, left);
-            if (cfdart_set_time(problem.grid(), t, n) || cfdart_run(problem.grid(), problem.splitting(), k, INFINITY,
-                                                                    sf.CFL, &t, &n)) {
+            if (cfdart_set_time(problem.grid(), t, n) || problem.run_steps(k, &t, &n)) {
                 if (err) *err = cfdart_last_error();
                 return 1;
             }
@@ -478,7 +490,6 @@ int run_simulation(const Settings& sf, std::string* err)
             if (std::fabs(t - problem.T()) < 1e-12) t = problem.T();
             std::cout << "[simulation] Time step " << n << " complete. Time t = " << t << "." << std::endl;
         }
-        problem.compute_dt(n, t);  // final mass record
         std::cout << "[simulation] Time stepping complete." << std::endl;
         problem.output(n, t, true);
     }

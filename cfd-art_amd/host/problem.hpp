@@ -54,7 +54,7 @@ public:
     void output(int n, double t, bool force);
     double compute_dt(int n, double t);
     int advance(int n, double dt);  // splitting_scheme::advance_timestep
-    int run_steps(int nsteps);      // device-resident loop for fixed-step mode
+    int run_steps(int nsteps, double* t, int* n);  // device-resident loop for fixed-step mode
 
     const cfdart_desc& desc() const { return d_; }
     double T() const { return T_; }
