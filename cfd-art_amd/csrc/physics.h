@@ -262,6 +262,7 @@ template <int M> struct Side {
     static constexpr int NC = Model<M>::NC;
     double U[NC];
     double rho, u, p, c;
+    double eps, k;  // Allaire: the mixture coefficients of z = U[5] (the PVRS speeds reuse them)
     bool ok;
 };
 
@@ -291,6 +292,7 @@ CFD_HD void side_from_cons(const Material& m, Side<ALLAIRE>& s)
     double p = MixEos::pressure(x, rho, e);
     s.rho = rho; s.u = u; s.p = p;
     s.c = MixEos::soundspeed(x, rho, p);
+    s.eps = x.eps; s.k = x.k;
     s.ok = U[0] >= 0.0 && U[1] >= 0.0 && e >= 0.0 && z >= 0.0 && z <= 1.0;
 }
 
@@ -306,6 +308,7 @@ CFD_HD void side_from_cons_mix(const Mix& x, Side<ALLAIRE>& s)
     double p = MixEos::pressure(x, rho, e);
     s.rho = rho; s.u = u; s.p = p;
     s.c = MixEos::soundspeed(x, rho, p);
+    s.eps = x.eps; s.k = x.k;
     s.ok = U[0] >= 0.0 && U[1] >= 0.0 && e >= 0.0 && z >= 0.0 && z <= 1.0;
 }
 
@@ -458,9 +461,14 @@ CFD_HD void hllc(const Material& m, int wave_speeds, const Side<M>& L, const Sid
         double rho_avg = 0.5 * (rhoL + rhoR);
         double c_avg = 0.5 * (cL + cR);
         double p_pvrs = smax(0.0, 0.5 * (pL + pR) + 0.5 * (uL - uR) * rho_avg * c_avg);
-        if (p_pvrs > pL) SL = uL - MixEos::shock_jump(mixture(m, L.U[NC - 1]), pL, rhoL, p_pvrs);
+        // shock_speed_jump evaluates the mixture of each side's z (A/headers/mixturemodel.hpp:42-52):
+        // the side already holds exactly those coefficients
+        Mix xL, xR;
+        xL.eps = L.eps; xL.k = L.k;
+        xR.eps = R.eps; xR.k = R.k;
+        if (p_pvrs > pL) SL = uL - MixEos::shock_jump(xL, pL, rhoL, p_pvrs);
         else SL = uL - cL;
-        if (p_pvrs > pR) SR = uR + MixEos::shock_jump(mixture(m, R.U[NC - 1]), pR, rhoR, p_pvrs);
+        if (p_pvrs > pR) SR = uR + MixEos::shock_jump(xR, pR, rhoR, p_pvrs);
         else SR = uR + cR;
     } else {
         double u_avg = 0.5 * (uL + uR);

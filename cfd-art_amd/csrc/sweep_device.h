@@ -50,6 +50,10 @@ template <int M> __device__ __forceinline__ void side_from_left(const Side<M>& s
     o.u = lane_from_left_or_self(s.u);
     o.p = lane_from_left_or_self(s.p);
     o.c = lane_from_left_or_self(s.c);
+    if (M == ALLAIRE) {
+        o.eps = lane_from_left_or_self(s.eps);
+        o.k = lane_from_left_or_self(s.k);
+    }
     o.ok = lane_from_left_or_self_i(s.ok ? 1 : 0) != 0;
 }
 
