@@ -2,6 +2,7 @@
 import math
 
 import numpy as np
+import pytest
 
 import cfdart
 from cfdart import serve
@@ -54,3 +55,24 @@ def test_state_checksum_host_restatement():
     t = s.copy()
     t[[3, 4], 5, 2] = t[[4, 3], 5, 2]
     assert cfdart.state_checksum(t) != c
+
+
+def test_serve_argument_errors_need_no_gpu():
+    L = serve._bind()
+    import ctypes as C
+    out = (serve.Request * 4)()
+    n = C.c_int()
+    assert L.cfdart_serve_trace(1, -1.0, 1.0, 0.5, 64, 128, 1, 2, out, 4, C.byref(n)) == -1   # rate
+    assert L.cfdart_serve_trace(1, 1.0, 1.0, 0.5, 128, 64, 1, 2, out, 4, C.byref(n)) == -1    # n_hi < n_lo
+    assert L.cfdart_serve_trace(1, 1.0, 1.0, 0.5, 64, 128, 5, 2, out, 4, C.byref(n)) == -1    # steps_hi < steps_lo
+    bad = serve.requests([(0.0, 7, 64, 10, 1)])
+    with pytest.raises(cfdart.CfdartError):
+        serve.setup(bad[0])  # unknown model
+    unsorted = serve.requests([(1.0, 0, 64, 10, 1), (0.5, 0, 64, 10, 2)])
+    res = (serve.RequestResult * 2)()
+    s = serve.ServeSummary()
+    o = serve.opts()
+    assert L.cfdart_serve_batched(unsorted, 2, C.byref(o), res, C.byref(s)) == -1           # arrivals not sorted
+    o2 = serve.opts(chunk_steps=7, frame_every=50)
+    ok = serve.requests([(0.0, 0, 64, 10, 1)])
+    assert L.cfdart_serve_batched(ok, 1, C.byref(o2), res, C.byref(s)) == -1                # chunk must divide frames
