@@ -11,6 +11,7 @@ MINMOD, VANLEER = 0, 1
 GARRICK, PVRS = 0, 1
 NAIVE, STRANG, ALTERNATING = 0, 1, 2
 MUSCL_HANCOCK, GODUNOV = 0, 1
+Y_TRANSPOSED = -1  # set_tile: the transposed y tile (CFDART_Y_TRANSPOSED)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 library_path = os.path.join(os.path.dirname(_HERE), "libcfdart.so")

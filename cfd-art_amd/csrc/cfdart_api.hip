@@ -195,7 +195,7 @@ int cfdart_create(const cfdart_desc* desc, cfdart_grid** out)
     g->pitch = (desc->nx + 31) / 32 * 32;  // 256-byte aligned rows
     g->plane = (long long)g->pitch * desc->ny;
     g->cur = 0;
-    g->yH = 32;
+    g->yH = CFDART_Y_TRANSPOSED;
     g->xrows = 4;
     g->buf[0] = g->buf[1] = nullptr;
     g->ctrl = nullptr;
@@ -553,7 +553,8 @@ int cfdart_set_strip_rows(cfdart_grid* g, int rows)
 
 int cfdart_set_tile(cfdart_grid* g, int y_strip_rows, int x_rows_per_wave)
 {
-    if (!g || y_strip_rows < 1 || y_strip_rows > 4096 || x_rows_per_wave < 1 || x_rows_per_wave > 256)
+    if (!g || (y_strip_rows < 1 && y_strip_rows != CFDART_Y_TRANSPOSED) || y_strip_rows > 4096 ||
+        x_rows_per_wave < 1 || x_rows_per_wave > 256)
         return fail(CFDART_E_ARG, "tile shape out of range");
     g->yH = y_strip_rows;
     g->xrows = x_rows_per_wave;

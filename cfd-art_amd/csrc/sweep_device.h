@@ -450,6 +450,121 @@ __device__ __forceinline__ void y_tile(const GridView& g, double dt, int bx, int
     }
 }
 
+// Transposed y-sweep tile: rows [y0-2, y0+62) x columns [x0, x0+YT_COLS) staged in LDS (one
+// 128-byte line per plane row, loaded and stored coalesced). Each wavefront runs the x sweep's
+// lane-parallel line computation down YT_COLS/4 of the columns -- lanes along y, neighbours
+// by DPP -- and overwrites its columns in LDS with the results; the workgroup then stores the
+// middle 60 rows. Same values as y_tile (same cell and face stages, rotated frame).
+template <int M, bool LAST>
+__device__ __forceinline__ void y_tile_t(const GridView& g, double dt, int x0, int y0, Tally& tl)
+{
+    constexpr int NC = Model<M>::NC;
+    constexpr int MX = Model<M>::MN, MY = Model<M>::MT;
+    __shared__ double sU[NC * 64 * YT_PAD];
+    const int tid = threadIdx.x;
+    {
+        const int col = tid % YT_COLS, rsub = tid / YT_COLS;
+        const int xs = min(x0 + col, g.nx - 1);
+#pragma unroll
+        for (int pass = 0; pass < 64 * YT_COLS / 256; ++pass) {
+            const int r = pass * (256 / YT_COLS) + rsub;
+            const int ys = mirror(y0 - 2 + r, g.ny);
+            const double* b = g.cur + (long long)ys * g.pitch + xs;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) sU[(k * 64 + r) * YT_PAD + col] = b[k * g.plane];
+        }
+    }
+    __syncthreads();
+    const int lane = tid & 63, w = tid >> 6;
+    const int y = y0 - 2 + lane;
+    const bool in_rows = lane >= 2 && lane < 62 && y < g.ny;
+    const bool face_row = (lane >= 2 && lane < 62 && y <= g.ny) || (lane == 62 && y == g.ny);
+    const bool neg = (y < 0 && g.bc[2]) || (y >= g.ny && g.bc[3]);
+    const double s = 0.5 * (dt / g.dx_half_col);
+    const double r = dt / g.dy;
+    for (int cc = 0; cc < YT_COLS / 4; ++cc) {
+        const int c = w + 4 * cc;
+        const bool col_ok = x0 + c < g.nx;
+        const bool store = in_rows && col_ok;
+        double raw[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) raw[k] = sU[(k * 64 + lane) * YT_PAD + c];
+        // ghost rows of a reflective bottom/top wall negate rho*v (E/sourcecode/euler_ppsweep.cpp:79,105);
+        // rotation (m_x, m_y) -> (m_y, -m_x) of E/sourcecode/euler_colsweep.cpp:118-124
+        if (neg) raw[MY] = -raw[MY];
+        double U[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) U[k] = raw[k];
+        U[MX] = raw[MY];
+        U[MY] = -raw[MX];
+
+        Prim<M> P;
+        P.from_cons(g.mat, U);
+        double Wm[NC], Wp[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) { Wm[k] = lane_from_left_or_self(P.W[k]); Wp[k] = lane_from_right_or_self(P.W[k]); }
+        CellStage<M> cs;
+        cell_stage<M>(g.mat, g.opt.limiter, s, P, Wm, Wp, cs);
+        Prim<M> Pa;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) Pa.W[k] = Wm[k];
+        Pa.c = lane_from_left(P.c);
+        double Ua[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) Ua[k] = lane_from_left(U[k]);
+        CellStage<M> a;
+        a.fR = lane_from_left_or_self_i(cs.fR ? 1 : 0) != 0;
+        side_from_left<M>(cs.R, a.R);
+        FaceOut<M> fo;
+        int fl = face_stage<M>(g.mat, g.opt, s, Pa, Ua, a, P, U, cs, fo);
+        double Fr[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) Fr[k] = lane_from_right(fo.F[k]);
+        double usr = lane_from_right(fo.ustar);
+        double zsr = M == ALLAIRE ? lane_from_right(fo.zstar) : 0.0;
+        double Un[NC];
+        bool bad;
+        update_cell<M>(g, r, U, fo.F, Fr, fo.ustar, usr, fo.zstar, zsr, Un, bad, store);
+        // rotate back (m_x, m_y) -> (-m_y, m_x): E/sourcecode/euler_colsweep.cpp:163-168
+        double t = Un[MX];
+        Un[MX] = -Un[MY];
+        Un[MY] = t;
+        bool rep = false;
+        if (M == EULER && store) rep = euler_repair(g.mat, raw, Un);
+        if (face_row && col_ok) { tl.slope += fl & 1; tl.evolve += (fl >> 1) & 1; }
+        if (store) {
+            tl.repair += rep;
+            tl.bad += bad;
+            if (LAST) {
+                double su, sv;
+                wave_speeds(g.mat, Un, su, sv, (Model<M>*)nullptr);
+                tl.su = smax(tl.su, su);
+                tl.sv = smax(tl.sv, sv);
+            }
+        }
+        // the column is this wavefront's alone: its results replace it
+#pragma unroll
+        for (int k = 0; k < NC; ++k) sU[(k * 64 + lane) * YT_PAD + c] = Un[k];
+    }
+    __syncthreads();
+    {
+        const int col = tid % YT_COLS, rsub = tid / YT_COLS;
+        const int x = x0 + col;
+        if (x < g.nx) {
+#pragma unroll
+            for (int pass = 0; pass < 64 * YT_COLS / 256; ++pass) {
+                const int rr = pass * (256 / YT_COLS) + rsub;
+                const int yy = y0 - 2 + rr;
+                if (rr >= 2 && rr < 62 && yy < g.ny) {
+                    double* out = g.nxt + (long long)yy * g.pitch + x;
+#pragma unroll
+                    for (int k = 0; k < NC; ++k) __builtin_nontemporal_store(sU[(k * 64 + rr) * YT_PAD + col], out + k * g.plane);
+                }
+            }
+        }
+    }
+}
+
 // One y-sweep tile of the first-order Godunov scheme: each lane marches rows [y0, y1) of its
 // column with a two-row window of stored states (faces y0 .. y1 need rows y0-1 .. y1).
 template <int M, bool LAST>

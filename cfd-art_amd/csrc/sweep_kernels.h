@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "cfdart.h"
 #include "physics.h"
 
 namespace cfd {
@@ -41,6 +42,11 @@ struct GridView {
 // Sweep-kernel launch geometry.
 constexpr int XTILE = 60;         // output columns per wavefront in the x sweep (64 lanes - 4 halo)
 constexpr int YBLOCK = 256;       // threads (columns) per y-sweep block
+constexpr int YT_COLS = 16;       // columns per transposed y-sweep tile (a 128-byte line per plane row)
+constexpr int YT_ROWS = 60;       // output rows per transposed y-sweep tile (64 lanes - 4 halo)
+constexpr int YT_PAD = YT_COLS + 1;  // LDS row pitch in doubles (conflict-free column reads)
+constexpr int Y_TRANSPOSED = CFDART_Y_TRANSPOSED;  // y-sweep tile selector: transposed LDS tile instead of a strip
+static_assert(256 % YT_COLS == 0 && YT_COLS % 4 == 0, "a 256-thread tile load covers whole rows; 4 waves share the columns");
 
 // Host-side launchers (sweep_kernels.hip).
 hipError_t launch_sweep(const GridView& g, int model, int axis, bool last, double dt_arg,

@@ -292,9 +292,9 @@ int cfdart_serve_one_at_a_time(const cfdart_request* reqs, int n, const cfdart_s
         }
         rc = d.model == CFDART_EULER ? cfdart_init_random_discs(g, &dp) : cfdart_init_shocked_bubbles(g, &bp);
         const int m = d.model;
-        if (!rc && (o->plain_y_strip_rows[m] > 0 || o->plain_x_rows_per_wave[m] > 0))
-            rc = cfdart_set_tile(g, o->plain_y_strip_rows[m] > 0 ? o->plain_y_strip_rows[m] : 32,
-                                 o->plain_x_rows_per_wave[m] > 0 ? o->plain_x_rows_per_wave[m] : 4);
+        if (!rc && (o->plain_y_strip_rows[m] != 0 || o->plain_x_rows_per_wave[m] != 0))
+            rc = cfdart_set_tile(g, o->plain_y_strip_rows[m] != 0 ? o->plain_y_strip_rows[m] : CFDART_Y_TRANSPOSED,
+                                 o->plain_x_rows_per_wave[m] != 0 ? o->plain_x_rows_per_wave[m] : 4);
         int taken = 0;
         const int every = o->frame_every > 0 ? o->frame_every : reqs[r].steps;
         while (!rc && taken < reqs[r].steps) {

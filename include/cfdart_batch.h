@@ -36,7 +36,8 @@ void cfdart_batch_destroy(cfdart_batch* b);
 
 /* Admits a grid: uploads host_state (format of cfdart.h), computes its first dt and adds its
  * tiles to the work-list. It then runs until it has taken max_steps steps or reached t >= T.
- * y_strip_rows / x_rows_per_wave: tile shape (0 = default). Returns CFDART_E_NOMEM when the
+ * y_strip_rows / x_rows_per_wave: tile shape as cfdart_set_tile, CFDART_Y_TRANSPOSED included
+ * (0 = default: 32-row strips, 8 rows per x wavefront). Returns CFDART_E_NOMEM when the
  * memory budget or the capacity would be exceeded (nothing is changed). slot_out receives the
  * grid's slot id. */
 int cfdart_batch_add(cfdart_batch* b, const cfdart_desc* desc, const double* host_state, int splitting,
