@@ -148,7 +148,7 @@ def bench_served(args):
     log(f"served: {len(reqs)} requests over {args.serve_window:.1f} s, {work / 1e9:.1f} G cell-updates offered "
         f"({work / args.serve_window / 1e9:.1f} G/s)")
     tile = {0: (64, 16), 1: (32, 16)}
-    plain = {0: (16, 4), 1: (8, 2)}
+    plain = {0: (cfdart.Y_TRANSPOSED, 4), 1: (cfdart.Y_TRANSPOSED_NARROW, 2)}
     o = serve.opts(mem_budget=200 * 10**9, capacity=8192, chunk_steps=10, frame_every=50,
                    frame_host_bytes=2 << 30, realtime=True, tile=tile, plain_tile=plain)
     rb, sb = serve.serve_batched(reqs, o)
@@ -267,7 +267,8 @@ def main():
             per_kernel[axis] = {"avg_us": 1e3 * ms / nl, "launches": nl,
                                 "gbs": bytes_per_launch / (ms / nl * 1e-3) / 1e9}
     dom = max(per_kernel, key=lambda a: per_kernel[a]["avg_us"] * per_kernel[a]["launches"])
-    kname = f"sweep_{dom}_kernel<{single['tag']}>"
+    transposed = dom == "y" and tuned["y_strip_rows"] < 0 and not args.strip_rows
+    kname = f"sweep_{'yt' if transposed else dom}_kernel<{single['tag']}>"
     achieved = per_kernel[dom]["gbs"]
     traffic = read_traffic(args.workload, dom)
 

@@ -11,7 +11,8 @@ MINMOD, VANLEER = 0, 1
 GARRICK, PVRS = 0, 1
 NAIVE, STRANG, ALTERNATING = 0, 1, 2
 MUSCL_HANCOCK, GODUNOV = 0, 1
-Y_TRANSPOSED = -1  # set_tile: the transposed y tile (CFDART_Y_TRANSPOSED)
+Y_TRANSPOSED = -1  # set_tile: the transposed y tile, 16 columns (CFDART_Y_TRANSPOSED)
+Y_TRANSPOSED_NARROW = -2  # 8 columns (CFDART_Y_TRANSPOSED_NARROW)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 library_path = os.path.join(os.path.dirname(_HERE), "libcfdart.so")

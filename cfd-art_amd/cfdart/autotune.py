@@ -2,9 +2,9 @@
 
 For each model and grid-size class it times the candidate tile shapes on the device
 (cfdart_time_tile: kernel time of one x sweep and one y sweep, HIP events, after warm-up) and
-keeps the fastest x-sweep rows-per-wavefront and y-sweep tile (a strip height, or the transposed
-tile L.Y_TRANSPOSED). The x and y choices are independent (separate kernels). Results are cached
-as JSON keyed by device name.
+keeps the fastest x-sweep rows-per-wavefront and y-sweep tile (a strip height, or one of the
+transposed tiles L.Y_TRANSPOSED and L.Y_TRANSPOSED_NARROW). The x and y choices are independent
+(separate kernels). Results are cached as JSON keyed by device name.
 """
 import json
 import math
@@ -51,7 +51,8 @@ def tune_one(model, n, reps=5, log=None, case=None):
     for yh in _around(min(ty, key=ty.get), 4, 2):
         if yh not in ty:
             ty[yh] = L.time_tile(d, yh, 4, reps, h)[1]
-    ty[L.Y_TRANSPOSED] = L.time_tile(d, L.Y_TRANSPOSED, 4, reps, h)[1]
+    for yt in (L.Y_TRANSPOSED, L.Y_TRANSPOSED_NARROW):
+        ty[yt] = L.time_tile(d, yt, 4, reps, h)[1]
     bx = min(tx, key=tx.get)
     by = min(ty, key=ty.get)
     if log:
@@ -79,7 +80,7 @@ def shape_for(table, model, nx, ny):
     return e["y_strip_rows"], e["x_rows_per_wave"]
 
 
-def tune_batch(batch, slots_by_model, steps=4, log=None, y_candidates=(16, 32, 64, 128, L.Y_TRANSPOSED),
+def tune_batch(batch, slots_by_model, steps=4, log=None, y_candidates=(16, 32, 64, 128),
                x_candidates=(4, 8, 16, 32)):
     """Tile shapes for a live batch, per model: coordinate descent (x rows per wavefront, then y
     tile) on the wall time of `steps` whole batch steps. The batch state advances during

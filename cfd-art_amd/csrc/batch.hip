@@ -112,8 +112,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void b
             const int xrows = bg->xrows;
             const int row0 = (tl.b * 4 + (threadIdx.x >> 6)) * xrows;
             x_tile<M, LAST>(g, dt, tl.a, row0, xrows, ta);
-        } else if (bg->yH == Y_TRANSPOSED) {
-            y_tile_t<M, LAST>(g, dt, tl.a * YT_COLS, tl.b * YT_ROWS, ta);
         } else {
             const int y0 = tl.b * bg->yH, y1 = min(y0 + bg->yH, bg->ny);
             y_tile<M, LAST>(g, dt, tl.a, y0, y1, ta);
@@ -251,8 +249,7 @@ double* pool_get(cfdart_batch* b, size_t bytes)
 // Estimated relative cost of a tile (LPT ordering: heavy tiles first).
 double tile_cost(const BatchGrid& g, int axis)
 {
-    if (axis == 0) return 60.0 * 4 * g.xrows;
-    return g.yH == cfd::Y_TRANSPOSED ? double(cfd::YT_COLS * cfd::YT_ROWS) : 256.0 * g.yH;
+    return axis == 0 ? 60.0 * 4 * g.xrows : 256.0 * g.yH;
 }
 
 int rebuild_tiles(cfdart_batch* b)
@@ -265,9 +262,7 @@ int rebuild_tiles(cfdart_batch* b)
         int ntx = (g.nx + cfd::XTILE - 1) / cfd::XTILE, nry = (g.ny + 4 * g.xrows - 1) / (4 * g.xrows);
         for (int a = 0; a < ntx; ++a)
             for (int r = 0; r < nry; ++r) out.push_back({tile_cost(g, 0), Tile{s, 0, a, r}});
-        const bool yt = g.yH == cfd::Y_TRANSPOSED;
-        const int bw = yt ? cfd::YT_COLS : cfd::YBLOCK, bh = yt ? cfd::YT_ROWS : g.yH;
-        int nbx = (g.nx + bw - 1) / bw, nsy = (g.ny + bh - 1) / bh;
+        int nbx = (g.nx + cfd::YBLOCK - 1) / cfd::YBLOCK, nsy = (g.ny + g.yH - 1) / g.yH;
         for (int a = 0; a < nbx; ++a)
             for (int r = 0; r < nsy; ++r) out.push_back({tile_cost(g, 1), Tile{s, 1, a, r}});
     }
@@ -450,8 +445,10 @@ int admit(cfdart_batch* b, const cfdart_desc* d, const double* host, const cfdar
     if (d->model != CFDART_EULER && d->model != CFDART_ALLAIRE) return bfail(CFDART_E_ARG, "bad model");
     if (d->nx < 2 || d->ny < 2 || d->ng != 2 || !(d->dx > 0) || !(d->dy > 0)) return bfail(CFDART_E_ARG, "bad grid");
     if (d->flux_solver != CFDART_MUSCL_HANCOCK) return bfail(CFDART_E_ARG, "batched grids use MUSCL-Hancock");
-    if ((yH < 0 && yH != CFDART_Y_TRANSPOSED) || yH > 4096 || xrows < 0 || xrows > 256)
-        return bfail(CFDART_E_ARG, "tile shape out of range");
+    // the persistent kernel runs y strips (a transposed y tile would need LDS staging that the
+    // kernel's two-wavefront occupancy cannot hide)
+    if (yH < 0 || yH > 4096 || xrows < 0 || xrows > 256)
+        return bfail(CFDART_E_ARG, "tile shape out of range (batched grids run y strips: y_strip_rows >= 1, 0 = default)");
     if (yH == 0) yH = 32;
     if (xrows == 0) xrows = 8;
     const int nc = d->model == CFDART_ALLAIRE ? 6 : 4;
@@ -679,8 +676,7 @@ int cfdart_batch_set_tile(cfdart_batch* b, int slot, int yH, int xrows)
 {
     int rc = check_slot(b, slot);
     if (rc) return rc;
-    if ((yH < 1 && yH != CFDART_Y_TRANSPOSED) || yH > 4096 || xrows < 1 || xrows > 256)
-        return bfail(CFDART_E_ARG, "tile shape out of range");
+    if (yH < 1 || yH > 4096 || xrows < 1 || xrows > 256) return bfail(CFDART_E_ARG, "tile shape out of range");
     BatchGrid& g = b->h_grids[slot];
     g.yH = yH;
     g.xrows = xrows;

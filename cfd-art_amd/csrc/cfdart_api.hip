@@ -553,7 +553,7 @@ int cfdart_set_strip_rows(cfdart_grid* g, int rows)
 
 int cfdart_set_tile(cfdart_grid* g, int y_strip_rows, int x_rows_per_wave)
 {
-    if (!g || (y_strip_rows < 1 && y_strip_rows != CFDART_Y_TRANSPOSED) || y_strip_rows > 4096 ||
+    if (!g || (y_strip_rows < 1 && !cfd::y_transposed(y_strip_rows)) || y_strip_rows > 4096 ||
         x_rows_per_wave < 1 || x_rows_per_wave > 256)
         return fail(CFDART_E_ARG, "tile shape out of range");
     g->yH = y_strip_rows;

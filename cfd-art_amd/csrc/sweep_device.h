@@ -450,28 +450,30 @@ __device__ __forceinline__ void y_tile(const GridView& g, double dt, int bx, int
     }
 }
 
-// Transposed y-sweep tile: rows [y0-2, y0+62) x columns [x0, x0+YT_COLS) staged in LDS (one
-// 128-byte line per plane row, loaded and stored coalesced). Each wavefront runs the x sweep's
-// lane-parallel line computation down YT_COLS/4 of the columns -- lanes along y, neighbours
-// by DPP -- and overwrites its columns in LDS with the results; the workgroup then stores the
-// middle 60 rows. Same values as y_tile (same cell and face stages, rotated frame).
-template <int M, bool LAST>
-__device__ __forceinline__ void y_tile_t(const GridView& g, double dt, int x0, int y0, Tally& tl)
+// Transposed y-sweep tile: rows [y0-2, y0+62) x columns [x0, x0+COLS) staged in the LDS array
+// sU (NC * 64 * (COLS+1) doubles; one 8*COLS-byte segment per plane row, loaded and stored
+// coalesced). Each wavefront runs the x sweep's lane-parallel line computation down COLS/4 of
+// the columns -- lanes along y, neighbours by DPP -- and overwrites its columns in LDS with the
+// results; the workgroup (256 threads) then stores the middle 60 rows. Same values as y_tile
+// (same cell and face stages, rotated frame).
+template <int M, bool LAST, int COLS>
+__device__ __forceinline__ void y_tile_t(const GridView& g, double dt, int x0, int y0, double* sU, Tally& tl)
 {
     constexpr int NC = Model<M>::NC;
     constexpr int MX = Model<M>::MN, MY = Model<M>::MT;
-    __shared__ double sU[NC * 64 * YT_PAD];
+    constexpr int PAD = COLS + 1;  // LDS row pitch in doubles: conflict-free column reads
+    static_assert(256 % COLS == 0 && COLS % 4 == 0, "a 256-thread tile load covers whole rows; 4 waves share the columns");
     const int tid = threadIdx.x;
     {
-        const int col = tid % YT_COLS, rsub = tid / YT_COLS;
+        const int col = tid % COLS, rsub = tid / COLS;
         const int xs = min(x0 + col, g.nx - 1);
 #pragma unroll
-        for (int pass = 0; pass < 64 * YT_COLS / 256; ++pass) {
-            const int r = pass * (256 / YT_COLS) + rsub;
+        for (int pass = 0; pass < 64 * COLS / 256; ++pass) {
+            const int r = pass * (256 / COLS) + rsub;
             const int ys = mirror(y0 - 2 + r, g.ny);
             const double* b = g.cur + (long long)ys * g.pitch + xs;
 #pragma unroll
-            for (int k = 0; k < NC; ++k) sU[(k * 64 + r) * YT_PAD + col] = b[k * g.plane];
+            for (int k = 0; k < NC; ++k) sU[(k * 64 + r) * PAD + col] = b[k * g.plane];
         }
     }
     __syncthreads();
@@ -482,13 +484,13 @@ __device__ __forceinline__ void y_tile_t(const GridView& g, double dt, int x0, i
     const bool neg = (y < 0 && g.bc[2]) || (y >= g.ny && g.bc[3]);
     const double s = 0.5 * (dt / g.dx_half_col);
     const double r = dt / g.dy;
-    for (int cc = 0; cc < YT_COLS / 4; ++cc) {
+    for (int cc = 0; cc < COLS / 4; ++cc) {
         const int c = w + 4 * cc;
         const bool col_ok = x0 + c < g.nx;
         const bool store = in_rows && col_ok;
         double raw[NC];
 #pragma unroll
-        for (int k = 0; k < NC; ++k) raw[k] = sU[(k * 64 + lane) * YT_PAD + c];
+        for (int k = 0; k < NC; ++k) raw[k] = sU[(k * 64 + lane) * PAD + c];
         // ghost rows of a reflective bottom/top wall negate rho*v (E/sourcecode/euler_ppsweep.cpp:79,105);
         // rotation (m_x, m_y) -> (m_y, -m_x) of E/sourcecode/euler_colsweep.cpp:118-124
         if (neg) raw[MY] = -raw[MY];
@@ -544,21 +546,21 @@ __device__ __forceinline__ void y_tile_t(const GridView& g, double dt, int x0, i
         }
         // the column is this wavefront's alone: its results replace it
 #pragma unroll
-        for (int k = 0; k < NC; ++k) sU[(k * 64 + lane) * YT_PAD + c] = Un[k];
+        for (int k = 0; k < NC; ++k) sU[(k * 64 + lane) * PAD + c] = Un[k];
     }
     __syncthreads();
     {
-        const int col = tid % YT_COLS, rsub = tid / YT_COLS;
+        const int col = tid % COLS, rsub = tid / COLS;
         const int x = x0 + col;
         if (x < g.nx) {
 #pragma unroll
-            for (int pass = 0; pass < 64 * YT_COLS / 256; ++pass) {
-                const int rr = pass * (256 / YT_COLS) + rsub;
+            for (int pass = 0; pass < 64 * COLS / 256; ++pass) {
+                const int rr = pass * (256 / COLS) + rsub;
                 const int yy = y0 - 2 + rr;
                 if (rr >= 2 && rr < 62 && yy < g.ny) {
                     double* out = g.nxt + (long long)yy * g.pitch + x;
 #pragma unroll
-                    for (int k = 0; k < NC; ++k) __builtin_nontemporal_store(sU[(k * 64 + rr) * YT_PAD + col], out + k * g.plane);
+                    for (int k = 0; k < NC; ++k) __builtin_nontemporal_store(sU[(k * 64 + rr) * PAD + col], out + k * g.plane);
                 }
             }
         }

@@ -42,11 +42,12 @@ struct GridView {
 // Sweep-kernel launch geometry.
 constexpr int XTILE = 60;         // output columns per wavefront in the x sweep (64 lanes - 4 halo)
 constexpr int YBLOCK = 256;       // threads (columns) per y-sweep block
-constexpr int YT_COLS = 16;       // columns per transposed y-sweep tile (a 128-byte line per plane row)
 constexpr int YT_ROWS = 60;       // output rows per transposed y-sweep tile (64 lanes - 4 halo)
-constexpr int YT_PAD = YT_COLS + 1;  // LDS row pitch in doubles (conflict-free column reads)
-constexpr int Y_TRANSPOSED = CFDART_Y_TRANSPOSED;  // y-sweep tile selector: transposed LDS tile instead of a strip
-static_assert(256 % YT_COLS == 0 && YT_COLS % 4 == 0, "a 256-thread tile load covers whole rows; 4 waves share the columns");
+// y-sweep tile selectors (the strip height's place): transposed LDS tiles of 16 or 8 columns
+constexpr int Y_TRANSPOSED = CFDART_Y_TRANSPOSED;
+constexpr int Y_TRANSPOSED_NARROW = CFDART_Y_TRANSPOSED_NARROW;
+constexpr bool y_transposed(int yH) { return yH == Y_TRANSPOSED || yH == Y_TRANSPOSED_NARROW; }
+constexpr int yt_cols(int yH) { return yH == Y_TRANSPOSED_NARROW ? 8 : 16; }
 
 // Host-side launchers (sweep_kernels.hip).
 hipError_t launch_sweep(const GridView& g, int model, int axis, bool last, double dt_arg,

@@ -63,25 +63,26 @@ __global__ __launch_bounds__(YBLOCK) void sweep_y_kernel(GridView g, double dt_a
     tl.flush(g.ctrl, LAST, blockIdx.x + blockIdx.y * gridDim.x);
 }
 
-// Transposed y sweep (MUSCL-Hancock): a workgroup stages 64 rows x 16 columns through LDS and
+// Transposed y sweep (MUSCL-Hancock): a workgroup stages 64 rows x COLS columns through LDS and
 // its four wavefronts run lanes along y; tiles in column-major order like sweep_y_kernel.
-template <int M, bool LAST>
+template <int M, bool LAST, int COLS>
 __global__ __launch_bounds__(256) void sweep_yt_kernel(GridView g, double dt_arg, double dt_factor, int use_ctrl)
 {
+    __shared__ double sU[Model<M>::NC * 64 * (COLS + 1)];
     const int b = blockIdx.x + blockIdx.y * gridDim.x;
     const int L = xcd_order(b, gridDim.x * gridDim.y);
     const int bx = L / gridDim.y, by = L % gridDim.y;
-    const int x0 = bx * YT_COLS, y0 = by * YT_ROWS;
+    const int x0 = bx * COLS, y0 = by * YT_ROWS;
     double dt;
     if (!sweep_dt(g, dt_arg, dt_factor, use_ctrl, dt)) {
-        if (threadIdx.x < YT_COLS) {
+        if (threadIdx.x < COLS) {
             const int x = x0 + threadIdx.x;
             copy_rows<M>(g, x, x < g.nx, y0, min(y0 + YT_ROWS, g.ny));
         }
         return;
     }
     Tally tl;
-    y_tile_t<M, LAST>(g, dt, x0, y0, tl);
+    y_tile_t<M, LAST, COLS>(g, dt, x0, y0, sU, tl);
     tl.flush(g.ctrl, LAST, b);
 }
 
@@ -230,9 +231,11 @@ static hipError_t launch_x(const GridView& g, double dt_arg, double f, int use_c
 template <int M, bool LAST, bool GOD>
 static hipError_t launch_y(const GridView& g, double dt_arg, double f, int use_ctrl, int H, hipStream_t st)
 {
-    if (H == Y_TRANSPOSED && !GOD) {
-        dim3 grid((g.nx + YT_COLS - 1) / YT_COLS, (g.ny + YT_ROWS - 1) / YT_ROWS);
-        hipLaunchKernelGGL((sweep_yt_kernel<M, LAST>), grid, dim3(256), 0, st, g, dt_arg, f, use_ctrl);
+    if (y_transposed(H) && !GOD) {
+        const int cols = yt_cols(H);
+        dim3 grid((g.nx + cols - 1) / cols, (g.ny + YT_ROWS - 1) / YT_ROWS);
+        if (cols == 16) hipLaunchKernelGGL((sweep_yt_kernel<M, LAST, 16>), grid, dim3(256), 0, st, g, dt_arg, f, use_ctrl);
+        else hipLaunchKernelGGL((sweep_yt_kernel<M, LAST, 8>), grid, dim3(256), 0, st, g, dt_arg, f, use_ctrl);
         return hipGetLastError();
     }
     if (H <= 0) H = 64;  // the first-order scheme has no transposed tile: a 64-row strip

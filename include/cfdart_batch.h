@@ -36,8 +36,8 @@ void cfdart_batch_destroy(cfdart_batch* b);
 
 /* Admits a grid: uploads host_state (format of cfdart.h), computes its first dt and adds its
  * tiles to the work-list. It then runs until it has taken max_steps steps or reached t >= T.
- * y_strip_rows / x_rows_per_wave: tile shape as cfdart_set_tile, CFDART_Y_TRANSPOSED included
- * (0 = default: 32-row strips, 8 rows per x wavefront). Returns CFDART_E_NOMEM when the
+ * y_strip_rows / x_rows_per_wave: tile shape as cfdart_set_tile, y strips only (0 = default:
+ * 32-row strips, 8 rows per x wavefront). Returns CFDART_E_NOMEM when the
  * memory budget or the capacity would be exceeded (nothing is changed). slot_out receives the
  * grid's slot id. */
 int cfdart_batch_add(cfdart_batch* b, const cfdart_desc* desc, const double* host_state, int splitting,
@@ -67,7 +67,7 @@ int cfdart_batch_checksum(cfdart_batch* b, int slot, unsigned long long* out);
 /* Fallback counters of a grid (see CFDART_CNT_*), 4 values; synchronous. */
 int cfdart_batch_stats(cfdart_batch* b, int slot, long long* counters);
 int cfdart_batch_sync(cfdart_batch* b);
-/* Changes a grid's tile shape (used by the autotuner between steps). */
+/* Changes a grid's tile shape (used by the autotuner between steps); y strips only. */
 int cfdart_batch_set_tile(cfdart_batch* b, int slot, int y_strip_rows, int x_rows_per_wave);
 /* Live grids, device bytes held by grid state, tiles in the work-list (each may be NULL; asking
  * for the tile count rebuilds a work-list left stale by admissions or removals). */
