@@ -174,21 +174,30 @@ __device__ __forceinline__ void x_tile(const GridView& g, double dt, int tx, int
     const bool face_counted = (lane >= 2 && lane < 62 && x <= g.nx) || (lane == 62 && x == g.nx);
     const int yend = min(row0 + nrows, g.ny);
 
-    // rows are independent: the next row's state is loaded while this one is computed
+    // rows are independent: for Euler the next row's state is loaded while this one is computed;
+    // Allaire, whose registers are the tighter limit, leaves the load latency to the other
+    // wavefronts (the prefetch registers would spill at its three wavefronts per SIMD)
+    constexpr bool PREFETCH = M == EULER;
     double Unext[NC];
-    if (row0 < yend) {
+    if (PREFETCH && row0 < yend) {
         const double* base = g.cur + (long long)row0 * g.pitch + xs;
 #pragma unroll
         for (int k = 0; k < NC; ++k) Unext[k] = base[k * g.plane];
     }
     for (int y = row0; y < yend; ++y) {
         double U[NC];
+        if constexpr (PREFETCH) {
 #pragma unroll
-        for (int k = 0; k < NC; ++k) U[k] = Unext[k];
-        if (y + 1 < yend) {
-            const double* base = g.cur + (long long)(y + 1) * g.pitch + xs;
+            for (int k = 0; k < NC; ++k) U[k] = Unext[k];
+            if (y + 1 < yend) {
+                const double* base = g.cur + (long long)(y + 1) * g.pitch + xs;
 #pragma unroll
-            for (int k = 0; k < NC; ++k) Unext[k] = base[k * g.plane];
+                for (int k = 0; k < NC; ++k) Unext[k] = base[k * g.plane];
+            }
+        } else {
+            const double* base = g.cur + (long long)y * g.pitch + xs;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) U[k] = base[k * g.plane];
         }
         if (neg) U[MN] = -U[MN];
 

@@ -17,8 +17,7 @@
 namespace cfd {
 
 // Occupancy of the MUSCL-Hancock x sweeps, as measured fastest: Allaire at 3 wavefronts per SIMD
-// (at most 168 VGPRs, a few registers spill; its natural 176-180 VGPRs allow 2), the Euler last
-// sweep at 4 (at most 128 VGPRs instead of its natural 134).
+// (at most 168 VGPRs), the Euler last sweep at 4 (at most 128 VGPRs instead of its natural 134).
 template <int M, bool LAST, bool GOD>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M == ALLAIRE && !GOD ? 3 : (LAST && !GOD ? 4 : 1)))) void sweep_x_kernel(GridView g, double dt_arg, double dt_factor, int use_ctrl,
                                                      int xrows)
