@@ -141,6 +141,7 @@ def bench_served(args):
     through the continuous-batching server, then the same requests one at a time with the plain
     solver. The window is shorter than the full 120 s so that the run stays within minutes; the
     offered load (about 31 G cell-updates/s) exceeds what one GPU serves, so requests queue."""
+    import cfdart
     from cfdart import serve
 
     reqs = serve.trace(seed=7, rate=50.0, duration=args.serve_window)
