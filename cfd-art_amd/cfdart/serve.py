@@ -16,7 +16,8 @@ class ServeOpts(C.Structure):
     _fields_ = [("mem_budget", C.c_size_t), ("capacity", C.c_int), ("chunk_steps", C.c_int),
                 ("frame_every", C.c_int), ("frame_host_bytes", C.c_size_t), ("realtime", C.c_int),
                 ("y_strip_rows", C.c_int * 2), ("x_rows_per_wave", C.c_int * 2),
-                ("plain_y_strip_rows", C.c_int * 2), ("plain_x_rows_per_wave", C.c_int * 2)]
+                ("plain_y_strip_rows", C.c_int * 2), ("plain_x_rows_per_wave", C.c_int * 2),
+                ("live_cells", C.c_longlong)]
 
 
 class RequestResult(C.Structure):
@@ -84,8 +85,9 @@ def setup(req):
 
 
 def opts(mem_budget=200 * 10**9, capacity=4096, chunk_steps=10, frame_every=50, frame_host_bytes=1 << 30,
-         realtime=True, tile=None, plain_tile=None):
+         realtime=True, tile=None, plain_tile=None, live_cells=0):
     o = ServeOpts()
+    o.live_cells = int(live_cells)
     o.mem_budget, o.capacity, o.chunk_steps, o.frame_every = int(mem_budget), capacity, chunk_steps, frame_every
     o.frame_host_bytes, o.realtime = int(frame_host_bytes), 1 if realtime else 0
     for m in (0, 1):

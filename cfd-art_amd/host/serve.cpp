@@ -167,6 +167,7 @@ int cfdart_serve_batched(const cfdart_request* reqs, int n, const cfdart_serve_o
     std::vector<int> owner(o->capacity, -1), steps(o->capacity), flags(o->capacity);
     std::deque<int> waiting;
     int next = 0, done = 0, live = 0;
+    long long live_cells = 0;
     size_t held = 0;
     std::vector<size_t> slot_bytes(o->capacity, 0);
     const Clock::time_point t0 = Clock::now();
@@ -187,6 +188,8 @@ int cfdart_serve_batched(const cfdart_request* reqs, int n, const cfdart_serve_o
             double cfl;
             int slot = -1;
             int e = cfdart_request_setup(&reqs[r], &d, &dp, &bp, &cfl);
+            const long long cells = (long long)reqs[r].n * reqs[r].n;
+            if (!e && o->live_cells > 0 && live > 0 && live_cells + cells > o->live_cells) break;
             if (!e) {
                 const int m = d.model;
                 e = cfdart_batch_add_generated(b, &d, m == CFDART_EULER ? &dp : nullptr,
@@ -211,6 +214,7 @@ int cfdart_serve_batched(const cfdart_request* reqs, int n, const cfdart_serve_o
             slot_bytes[slot] = bytes - held;
             held = bytes;
             ++live;
+            live_cells += cells;
         }
         sum->max_live = std::max(sum->max_live, live);
         sum->peak_state_bytes = std::max(sum->peak_state_bytes, (double)held);
@@ -255,6 +259,7 @@ int cfdart_serve_batched(const cfdart_request* reqs, int n, const cfdart_serve_o
                 slot_bytes[s] = 0;
                 owner[s] = -1;
                 --live;
+                live_cells -= (long long)reqs[r].n * reqs[r].n;
                 ++done;
             }
         }

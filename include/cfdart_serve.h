@@ -64,6 +64,9 @@ typedef struct cfdart_serve_opts {
     int plain_y_strip_rows[2]; /* one-at-a-time tile shape per model, as cfdart_set_tile
                                   (transposed y tiles included); 0 = default */
     int plain_x_rows_per_wave[2];
+    long long live_cells;    /* batched: admit while the live grids hold at most this many cells
+                                (enough to fill the GPU; the rest wait first come first served,
+                                which shortens each admitted request's latency); 0 = no limit */
 } cfdart_serve_opts;
 
 typedef struct cfdart_request_result {

@@ -104,3 +104,24 @@ def test_request_larger_than_budget_is_refused(gpu):
     rb, _ = serve.serve_batched(reqs, o)
     assert rb[0].status == 0 and rb[2].status == 0
     assert rb[1].status == -2 and rb[1].steps_done == 0
+
+
+def test_live_cell_cap(gpu):
+    """Admission capped by live cells: the cap holds whenever more than one grid is live, every
+    request still completes, and the results are bitwise those of the uncapped server."""
+    reqs = _mixed_requests()
+    cap = 3 * 160 * 160
+    o = serve.opts(capacity=64, chunk_steps=10, frame_every=0, realtime=False)
+    oc = serve.opts(capacity=64, chunk_steps=10, frame_every=0, realtime=False, live_cells=cap)
+    rb, sb = serve.serve_batched(reqs, o)
+    rc, sc = serve.serve_batched(reqs, oc)
+    print(f"max live uncapped {sb.max_live}, capped {sc.max_live}", flush=True)
+    assert sc.max_live < sb.max_live
+    for a, b in zip(rb, rc):
+        assert a.status == 0 and b.status == 0 and a.checksum == b.checksum
+    # every admission happened at a moment when the grids live then, plus the new one, fit the cap
+    # (or the new one was alone)
+    ivals = sorted((r.admitted, r.finished, q.n * q.n) for q, r in zip(reqs, rc))
+    for t, _, cells in ivals:
+        live = [c for a, f, c in ivals if a < t < f]
+        assert not live or sum(live) + cells <= cap
