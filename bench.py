@@ -65,8 +65,9 @@ def cpu_baseline(steps):
 
 
 def bench_batched(args):
-    """Batched workload: all grids advance together (one persistent launch per sweep per model,
-    a captured graph per step) versus the same grids one at a time with the plain solver."""
+    """Batched workload: all grids advance together (per sweep phase one x and one y launch per
+    model over the tiles of every grid, a captured graph per step) versus the same grids one at a
+    time with the plain solver."""
     import numpy as np
     import cfdart
 
@@ -148,7 +149,7 @@ def bench_served(args):
     work = float(sum(q.n * q.n * q.steps for q in reqs))
     log(f"served: {len(reqs)} requests over {args.serve_window:.1f} s, {work / 1e9:.1f} G cell-updates offered "
         f"({work / args.serve_window / 1e9:.1f} G/s)")
-    tile = {0: (64, 16), 1: (32, 16)}
+    tile = {0: (cfdart.Y_TRANSPOSED, 8), 1: (cfdart.Y_TRANSPOSED, 4)}
     plain = {0: (cfdart.Y_TRANSPOSED, 4), 1: (cfdart.Y_TRANSPOSED_NARROW, 2)}
     o = serve.opts(mem_budget=200 * 10**9, capacity=8192, chunk_steps=10, frame_every=50,
                    frame_host_bytes=2 << 30, realtime=True, tile=tile, plain_tile=plain)

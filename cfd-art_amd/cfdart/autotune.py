@@ -80,8 +80,8 @@ def shape_for(table, model, nx, ny):
     return e["y_strip_rows"], e["x_rows_per_wave"]
 
 
-def tune_batch(batch, slots_by_model, steps=4, log=None, y_candidates=(16, 32, 64, 128),
-               x_candidates=(4, 8, 16, 32)):
+def tune_batch(batch, slots_by_model, steps=4, log=None, y_candidates=(L.Y_TRANSPOSED,),
+               x_candidates=(2, 4, 8, 16)):
     """Tile shapes for a live batch, per model: coordinate descent (x rows per wavefront, then y
     tile) on the wall time of `steps` whole batch steps. The batch state advances during
     tuning, so callers tune on a copy or before the timed work. Returns {model: (yH, xrows)}."""
@@ -90,7 +90,7 @@ def tune_batch(batch, slots_by_model, steps=4, log=None, y_candidates=(16, 32, 6
     for model, slots in slots_by_model.items():
         if not slots:
             continue
-        cur = [32, 8]
+        cur = [y_candidates[0], 8]
 
         def measure(yh, xr):
             for s in slots:

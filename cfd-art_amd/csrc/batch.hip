@@ -1,6 +1,6 @@
-// Multi-grid execution (include/cfdart_batch.h): a persistent kernel per sweep over a device
-// work-list of tiles from every live grid, per-grid time-loop state on the device, and a HIP
-// graph per step.
+// Multi-grid execution (include/cfdart_batch.h): per sweep phase, an x-sweep and a y-sweep
+// kernel per model over device work-lists of tiles from every live grid, per-grid time-loop state
+// on the device, and a HIP graph per step.
 //
 // Buffer roles: at a step boundary each grid's state is in buf[0]; the first sweep of a step
 // reads buf[0] and writes buf[1], the second reads buf[1] and writes buf[0]. A grid whose loop
@@ -44,14 +44,13 @@ struct BatchGrid {
 };
 
 struct Tile {
-    int grid, axis, a, b;  // x: (column tile, row group); y: (column block, strip)
+    int grid, axis, a, b;  // x: (column tile, row group); y: (column block, row block)
 };
 
-// Work-list heads: one queue per model (Euler and Allaire tiles run in separate kernels so each
-// keeps its own register budget) and per sweep phase.
+// Work-list lengths, per model and axis (each list has its own kernel; the launch covers the
+// list's capacity and workgroups past its length exit at once).
 struct BatchQueue {
-    unsigned int head[2][2];  // [model][phase]
-    int ntiles[2];            // [model]
+    int ntiles[2][2];  // [model][axis]
 };
 
 __device__ __forceinline__ GridView batch_view(BatchGrid* bg, int phase)
@@ -73,64 +72,54 @@ __device__ __forceinline__ GridView batch_view(BatchGrid* bg, int phase)
     return g;
 }
 
-// One sweep phase of every live grid of model M. Workgroups pull tiles from the queue until
-// it is empty, so uneven grids balance across the chip.
-template <int M, bool LAST>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void batch_sweep_kernel(
-    BatchGrid* grids, const Tile* tiles, BatchQueue* q, int phase)
+// The sweep axis of a grid in a phase, or -1 when the grid does not run (dead or finished).
+__device__ __forceinline__ int batch_axis(const BatchGrid* bg, int phase)
 {
-    __shared__ int s_t, s_axis;
-    for (;;) {
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            // one thread claims tiles until it finds work: tiles of finished grids, and tiles of
-            // the other axis (a grid's axis per phase follows its own step parity), are skipped
-            // without synchronising the workgroup
-            int t, axis = 0;
-            for (;;) {
-                t = (int)atomicAdd(&q->head[M][phase], 1u);
-                if (t >= q->ntiles[M]) { t = -1; break; }
-                const Tile tc = tiles[t];
-                const BatchGrid* gc = grids + tc.grid;
-                if (!gc->live || !gc->ctrl.active) continue;
-                axis = (gc->splitting == 2 && (gc->ctrl.n & 1)) ? 1 - phase : phase;
-                if (tc.axis == axis) break;
-            }
-            s_t = t;
-            s_axis = axis;
-        }
-        __syncthreads();
-        const int t = s_t;
-        if (t < 0) break;
-        const Tile tl = tiles[t];
-        BatchGrid* bg = grids + tl.grid;
-        const int axis = s_axis;
-        const GridView g = batch_view(bg, phase);
-        const double dt = bg->ctrl.dt;
-        Tally ta;
-        if (axis == 0) {
-            const int xrows = bg->xrows;
-            const int row0 = (tl.b * 4 + (threadIdx.x >> 6)) * xrows;
-            x_tile<M, LAST>(g, dt, tl.a, row0, xrows, ta);
-        } else {
-            const int y0 = tl.b * bg->yH, y1 = min(y0 + bg->yH, bg->ny);
-            y_tile<M, LAST>(g, dt, tl.a, y0, y1, ta);
-        }
-        ta.flush(&bg->ctrl, LAST, t);
-    }
+    if (!bg->live || !bg->ctrl.active) return -1;
+    return (bg->splitting == 2 && (bg->ctrl.n & 1)) ? 1 - phase : phase;
+}
+
+// x sweep of one phase over the x tiles of every live grid of model M (one tile per workgroup,
+// the tile of sweep_x_kernel). Tiles of grids on the other axis this phase exit at once.
+template <int M, bool LAST>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M == ALLAIRE ? 3 : (LAST ? 4 : 1)))) void
+batch_x_kernel(BatchGrid* grids, const Tile* tiles, const BatchQueue* q, int phase)
+{
+    const int t = blockIdx.x;
+    if (t >= q->ntiles[M][0]) return;
+    const Tile tl = tiles[t];
+    BatchGrid* bg = grids + tl.grid;
+    if (batch_axis(bg, phase) != 0) return;
+    const GridView g = batch_view(bg, phase);
+    const int xrows = bg->xrows;
+    Tally ta;
+    x_tile<M, LAST>(g, bg->ctrl.dt, tl.a, (tl.b * 4 + (threadIdx.x >> 6)) * xrows, xrows, ta);
+    ta.flush(&bg->ctrl, LAST, t);
+}
+
+// y sweep of one phase over the y tiles of every live grid of model M: the transposed tile of
+// sweep_yt_kernel (64 rows x 16 columns through LDS, lanes along y).
+template <int M, bool LAST>
+__global__ __launch_bounds__(256) void batch_y_kernel(BatchGrid* grids, const Tile* tiles, const BatchQueue* q,
+                                                      int phase)
+{
+    __shared__ double sU[Model<M>::NC * 64 * (BATCH_YT_COLS + 1)];
+    const int t = blockIdx.x;
+    if (t >= q->ntiles[M][1]) return;
+    const Tile tl = tiles[t];
+    BatchGrid* bg = grids + tl.grid;
+    if (batch_axis(bg, phase) != 1) return;
+    const GridView g = batch_view(bg, phase);
+    Tally ta;
+    y_tile_t<M, LAST, BATCH_YT_COLS>(g, bg->ctrl.dt, tl.a * BATCH_YT_COLS, tl.b * YT_ROWS, sU, ta);
+    ta.flush(&bg->ctrl, LAST, t);
 }
 
 // End of a step for every live grid (E/headers/simulation.hpp:56-58 per grid, plus the
-// step budget), and the queue reset for the next step.
-__global__ void batch_finalize_kernel(BatchGrid* grids, int cap, BatchQueue* q, int* status)
+// step budget).
+__global__ void batch_finalize_kernel(BatchGrid* grids, int cap, int* status)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        q->head[0][0] = 0;
-        q->head[0][1] = 0;
-        q->head[1][0] = 0;
-        q->head[1][1] = 0;
-    }
     if (i >= cap) return;
     BatchGrid* bg = grids + i;
     Ctrl* c = &bg->ctrl;
@@ -149,13 +138,9 @@ __global__ void batch_finalize_kernel(BatchGrid* grids, int cap, BatchQueue* q, 
     status[2 * i + 1] = (c->active ? 1 : 0) | (c->error ? 2 : 0);
 }
 
-__global__ void batch_reset_queue_kernel(BatchQueue* q, int ntiles_euler, int ntiles_allaire)
+__global__ void batch_set_lengths_kernel(BatchQueue* q, BatchQueue lengths)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        for (int m = 0; m < 2; ++m) q->head[m][0] = q->head[m][1] = 0;
-        q->ntiles[0] = ntiles_euler;
-        q->ntiles[1] = ntiles_allaire;
-    }
+    if (threadIdx.x == 0 && blockIdx.x == 0) *q = lengths;
 }
 
 // After the reduction of a newly admitted grid: its first dt and loop test.
@@ -189,9 +174,9 @@ struct cfdart_batch {
     std::vector<BatchGrid> h_grids;
     std::vector<char> live;
     std::vector<size_t> bytes;
-    Tile* d_tiles[2];  // per model
-    size_t tile_cap[2];
-    int ntile[2];
+    Tile* d_tiles[2][2];  // per model and axis
+    size_t tile_cap[2][2];
+    int ntile[2][2];
     BatchQueue* d_q;
     double* d_partials;
     int nred;
@@ -199,7 +184,6 @@ struct cfdart_batch {
     hipEvent_t fork_ev, join_ev;
     hipGraphExec_t exec;
     bool have_graph;
-    int nblocks[2];
     int ntiles;
     std::multimap<size_t, double*> pool;  // freed state buffers for reuse, by size
     int* d_status;           // per slot: steps taken, flags (1 running, 2 error)
@@ -246,46 +230,50 @@ double* pool_get(cfdart_batch* b, size_t bytes)
     return p;
 }
 
-// Estimated relative cost of a tile (LPT ordering: heavy tiles first).
-double tile_cost(const BatchGrid& g, int axis)
-{
-    return axis == 0 ? 60.0 * 4 * g.xrows : 256.0 * g.yH;
-}
+// Estimated relative cost of an x tile (LPT ordering: heavy tiles first; y tiles are all alike).
+double tile_cost(const BatchGrid& g) { return 60.0 * 4 * g.xrows; }
 
 int rebuild_tiles(cfdart_batch* b)
 {
-    std::vector<std::pair<double, Tile>> v[2];
+    std::vector<std::pair<double, Tile>> v[2][2];
     for (int s = 0; s < b->cap; ++s) {
         if (!b->live[s]) continue;
         const BatchGrid& g = b->h_grids[s];
-        auto& out = v[g.model == cfd::ALLAIRE ? 1 : 0];
-        int ntx = (g.nx + cfd::XTILE - 1) / cfd::XTILE, nry = (g.ny + 4 * g.xrows - 1) / (4 * g.xrows);
+        const int m = g.model == cfd::ALLAIRE ? 1 : 0;
+        const int ntx = (g.nx + cfd::XTILE - 1) / cfd::XTILE, nry = (g.ny + 4 * g.xrows - 1) / (4 * g.xrows);
         for (int a = 0; a < ntx; ++a)
-            for (int r = 0; r < nry; ++r) out.push_back({tile_cost(g, 0), Tile{s, 0, a, r}});
-        int nbx = (g.nx + cfd::YBLOCK - 1) / cfd::YBLOCK, nsy = (g.ny + g.yH - 1) / g.yH;
+            for (int r = 0; r < nry; ++r) v[m][0].push_back({tile_cost(g), Tile{s, 0, a, r}});
+        // column blocks in the outer loop: tiles sharing halo rows are consecutive
+        const int nbx = (g.nx + cfd::BATCH_YT_COLS - 1) / cfd::BATCH_YT_COLS, nby = (g.ny + cfd::YT_ROWS - 1) / cfd::YT_ROWS;
         for (int a = 0; a < nbx; ++a)
-            for (int r = 0; r < nsy; ++r) out.push_back({tile_cost(g, 1), Tile{s, 1, a, r}});
+            for (int r = 0; r < nby; ++r) v[m][1].push_back({0.0, Tile{s, 1, a, r}});
     }
     b->ntiles = 0;
-    for (int m = 0; m < 2; ++m) {
-        std::stable_sort(v[m].begin(), v[m].end(),
-                         [](const std::pair<double, Tile>& x, const std::pair<double, Tile>& y) { return x.first > y.first; });
-        std::vector<Tile> t(v[m].size());
-        for (size_t i = 0; i < v[m].size(); ++i) t[i] = v[m][i].second;
-        if (t.size() > b->tile_cap[m]) {
-            BCHK(hipStreamSynchronize(b->st));
-            BCHK(hipStreamSynchronize(b->st2));
-            if (b->d_tiles[m]) BCHK(hipFree(b->d_tiles[m]));
-            b->tile_cap[m] = std::max(t.size(), 2 * b->tile_cap[m]);
-            BCHK(hipMalloc(&b->d_tiles[m], sizeof(Tile) * b->tile_cap[m]));
-            b->have_graph = false;  // the graph holds the tile pointers
+    BatchQueue lengths{};
+    for (int m = 0; m < 2; ++m)
+        for (int ax = 0; ax < 2; ++ax) {
+            auto& l = v[m][ax];
+            std::stable_sort(l.begin(), l.end(), [](const std::pair<double, Tile>& x, const std::pair<double, Tile>& y) {
+                return x.first > y.first;
+            });
+            std::vector<Tile> t(l.size());
+            for (size_t i = 0; i < l.size(); ++i) t[i] = l[i].second;
+            if (t.size() > b->tile_cap[m][ax]) {
+                BCHK(hipStreamSynchronize(b->st));
+                BCHK(hipStreamSynchronize(b->st2));
+                if (b->d_tiles[m][ax]) BCHK(hipFree(b->d_tiles[m][ax]));
+                b->tile_cap[m][ax] = std::max(t.size(), 2 * b->tile_cap[m][ax]);
+                BCHK(hipMalloc(&b->d_tiles[m][ax], sizeof(Tile) * b->tile_cap[m][ax]));
+                b->have_graph = false;  // the graph holds the list pointers and launch sizes
+            }
+            if (!t.empty())
+                BCHK(hipMemcpyAsync(b->d_tiles[m][ax], t.data(), sizeof(Tile) * t.size(), hipMemcpyHostToDevice,
+                                    b->st));
+            b->ntile[m][ax] = (int)t.size();
+            lengths.ntiles[m][ax] = (int)t.size();
+            b->ntiles += (int)t.size();
         }
-        if (!t.empty())
-            BCHK(hipMemcpyAsync(b->d_tiles[m], t.data(), sizeof(Tile) * t.size(), hipMemcpyHostToDevice, b->st));
-        b->ntile[m] = (int)t.size();
-        b->ntiles += (int)t.size();
-    }
-    hipLaunchKernelGGL(cfd::batch_reset_queue_kernel, dim3(1), dim3(64), 0, b->st, b->d_q, b->ntile[0], b->ntile[1]);
+    hipLaunchKernelGGL(cfd::batch_set_lengths_kernel, dim3(1), dim3(64), 0, b->st, b->d_q, lengths);
     BCHK(hipGetLastError());
     return 0;
 }
@@ -299,26 +287,30 @@ int capture_graph(cfdart_batch* b)
         b->have_graph = false;
     }
     for (int m = 0; m < 2; ++m)
-        if (!b->d_tiles[m]) {
-            b->tile_cap[m] = 1;
-            BCHK(hipMalloc(&b->d_tiles[m], sizeof(Tile)));
-        }
+        for (int ax = 0; ax < 2; ++ax)
+            if (!b->d_tiles[m][ax]) {
+                b->tile_cap[m][ax] = 1;
+                BCHK(hipMalloc(&b->d_tiles[m][ax], sizeof(Tile)));
+            }
+    auto nb = [&](int m, int ax) { return dim3((unsigned)b->tile_cap[m][ax]); };
     hipGraph_t graph;
     BCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
     BCHK(hipEventRecord(b->fork_ev, b->st));
     BCHK(hipStreamWaitEvent(b->st2, b->fork_ev, 0));
-    hipLaunchKernelGGL((cfd::batch_sweep_kernel<cfd::EULER, false>), dim3(b->nblocks[0]), dim3(256), 0, b->st,
-                       b->d_grids, b->d_tiles[0], b->d_q, 0);
-    hipLaunchKernelGGL((cfd::batch_sweep_kernel<cfd::EULER, true>), dim3(b->nblocks[0]), dim3(256), 0, b->st,
-                       b->d_grids, b->d_tiles[0], b->d_q, 1);
-    hipLaunchKernelGGL((cfd::batch_sweep_kernel<cfd::ALLAIRE, false>), dim3(b->nblocks[1]), dim3(256), 0, b->st2,
-                       b->d_grids, b->d_tiles[1], b->d_q, 0);
-    hipLaunchKernelGGL((cfd::batch_sweep_kernel<cfd::ALLAIRE, true>), dim3(b->nblocks[1]), dim3(256), 0, b->st2,
-                       b->d_grids, b->d_tiles[1], b->d_q, 1);
+    using cfd::ALLAIRE;
+    using cfd::EULER;
+    hipLaunchKernelGGL((cfd::batch_x_kernel<EULER, false>), nb(0, 0), dim3(256), 0, b->st, b->d_grids, b->d_tiles[0][0], b->d_q, 0);
+    hipLaunchKernelGGL((cfd::batch_y_kernel<EULER, false>), nb(0, 1), dim3(256), 0, b->st, b->d_grids, b->d_tiles[0][1], b->d_q, 0);
+    hipLaunchKernelGGL((cfd::batch_x_kernel<EULER, true>), nb(0, 0), dim3(256), 0, b->st, b->d_grids, b->d_tiles[0][0], b->d_q, 1);
+    hipLaunchKernelGGL((cfd::batch_y_kernel<EULER, true>), nb(0, 1), dim3(256), 0, b->st, b->d_grids, b->d_tiles[0][1], b->d_q, 1);
+    hipLaunchKernelGGL((cfd::batch_x_kernel<ALLAIRE, false>), nb(1, 0), dim3(256), 0, b->st2, b->d_grids, b->d_tiles[1][0], b->d_q, 0);
+    hipLaunchKernelGGL((cfd::batch_y_kernel<ALLAIRE, false>), nb(1, 1), dim3(256), 0, b->st2, b->d_grids, b->d_tiles[1][1], b->d_q, 0);
+    hipLaunchKernelGGL((cfd::batch_x_kernel<ALLAIRE, true>), nb(1, 0), dim3(256), 0, b->st2, b->d_grids, b->d_tiles[1][0], b->d_q, 1);
+    hipLaunchKernelGGL((cfd::batch_y_kernel<ALLAIRE, true>), nb(1, 1), dim3(256), 0, b->st2, b->d_grids, b->d_tiles[1][1], b->d_q, 1);
     BCHK(hipEventRecord(b->join_ev, b->st2));
     BCHK(hipStreamWaitEvent(b->st, b->join_ev, 0));
     hipLaunchKernelGGL(cfd::batch_finalize_kernel, dim3((b->cap + 255) / 256), dim3(256), 0, b->st, b->d_grids,
-                       b->cap, b->d_q, b->d_status);
+                       b->cap, b->d_status);
     BCHK(hipStreamEndCapture(b->st, &graph));
     hipError_t e = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
@@ -350,9 +342,12 @@ int cfdart_batch_create(int capacity, size_t mem_budget_bytes, cfdart_batch** ou
     b->h_grids.assign(capacity, BatchGrid{});
     b->live.assign(capacity, 0);
     b->bytes.assign(capacity, 0);
-    b->d_tiles[0] = b->d_tiles[1] = nullptr;
-    b->tile_cap[0] = b->tile_cap[1] = 0;
-    b->ntile[0] = b->ntile[1] = 0;
+    for (int m = 0; m < 2; ++m)
+        for (int ax = 0; ax < 2; ++ax) {
+            b->d_tiles[m][ax] = nullptr;
+            b->tile_cap[m][ax] = 0;
+            b->ntile[m][ax] = 0;
+        }
     b->have_graph = false;
     b->ntiles = 0;
     b->d_status = nullptr;
@@ -381,13 +376,6 @@ int cfdart_batch_create(int capacity, size_t mem_budget_bytes, cfdart_batch** ou
     (void)hipMemset(b->d_grids, 0, sizeof(BatchGrid) * capacity);
     (void)hipMemset(b->d_status, 0, sizeof(int) * 2 * capacity);
     (void)hipMemset(b->d_q, 0, sizeof(BatchQueue));
-    // persistent grid: as many workgroups as can be resident at once
-    int dev = 0, cus = 0, per_cu[2] = {0, 0};
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[0], cfd::batch_sweep_kernel<cfd::EULER, true>, 256, 0);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[1], cfd::batch_sweep_kernel<cfd::ALLAIRE, true>, 256, 0);
-    for (int m = 0; m < 2; ++m) b->nblocks[m] = std::max(1, cus * std::max(1, per_cu[m]));
     if (hipDeviceSynchronize() != hipSuccess) {
         cfdart_batch_destroy(b);
         return bfail(CFDART_E_HIP, "device initialisation failed");
@@ -410,7 +398,8 @@ void cfdart_batch_destroy(cfdart_batch* b)
     for (auto& kv : b->pool) (void)hipFree(kv.second);
     if (b->have_graph) (void)hipGraphExecDestroy(b->exec);
     for (int m = 0; m < 2; ++m)
-        if (b->d_tiles[m]) (void)hipFree(b->d_tiles[m]);
+        for (int ax = 0; ax < 2; ++ax)
+            if (b->d_tiles[m][ax]) (void)hipFree(b->d_tiles[m][ax]);
     if (b->d_grids) (void)hipFree(b->d_grids);
     if (b->d_q) (void)hipFree(b->d_q);
     if (b->d_partials) (void)hipFree(b->d_partials);
@@ -445,11 +434,11 @@ int admit(cfdart_batch* b, const cfdart_desc* d, const double* host, const cfdar
     if (d->model != CFDART_EULER && d->model != CFDART_ALLAIRE) return bfail(CFDART_E_ARG, "bad model");
     if (d->nx < 2 || d->ny < 2 || d->ng != 2 || !(d->dx > 0) || !(d->dy > 0)) return bfail(CFDART_E_ARG, "bad grid");
     if (d->flux_solver != CFDART_MUSCL_HANCOCK) return bfail(CFDART_E_ARG, "batched grids use MUSCL-Hancock");
-    // the persistent kernel runs y strips (a transposed y tile would need LDS staging that the
-    // kernel's two-wavefront occupancy cannot hide)
-    if (yH < 0 || yH > 4096 || xrows < 0 || xrows > 256)
-        return bfail(CFDART_E_ARG, "tile shape out of range (batched grids run y strips: y_strip_rows >= 1, 0 = default)");
-    if (yH == 0) yH = 32;
+    // the y sweep of a batch runs the 16-column transposed tile
+    if ((yH != 0 && yH != CFDART_Y_TRANSPOSED) || xrows < 0 || xrows > 256)
+        return bfail(CFDART_E_ARG, "tile shape out of range (batched grids: y_strip_rows 0 or CFDART_Y_TRANSPOSED, "
+                                   "x_rows_per_wave in [0, 256])");
+    yH = CFDART_Y_TRANSPOSED;
     if (xrows == 0) xrows = 8;
     const int nc = d->model == CFDART_ALLAIRE ? 6 : 4;
     const int pitch = (d->nx + 31) / 32 * 32;
@@ -676,7 +665,9 @@ int cfdart_batch_set_tile(cfdart_batch* b, int slot, int yH, int xrows)
 {
     int rc = check_slot(b, slot);
     if (rc) return rc;
-    if (yH < 1 || yH > 4096 || xrows < 1 || xrows > 256) return bfail(CFDART_E_ARG, "tile shape out of range");
+    if ((yH != 0 && yH != CFDART_Y_TRANSPOSED) || xrows < 1 || xrows > 256)
+        return bfail(CFDART_E_ARG, "tile shape out of range (batched grids: y_strip_rows 0 or CFDART_Y_TRANSPOSED)");
+    yH = CFDART_Y_TRANSPOSED;
     BatchGrid& g = b->h_grids[slot];
     g.yH = yH;
     g.xrows = xrows;

@@ -48,6 +48,7 @@ constexpr int Y_TRANSPOSED = CFDART_Y_TRANSPOSED;
 constexpr int Y_TRANSPOSED_NARROW = CFDART_Y_TRANSPOSED_NARROW;
 constexpr bool y_transposed(int yH) { return yH == Y_TRANSPOSED || yH == Y_TRANSPOSED_NARROW; }
 constexpr int yt_cols(int yH) { return yH == Y_TRANSPOSED_NARROW ? 8 : 16; }
+constexpr int BATCH_YT_COLS = 16;  // the batch's y tile: the 16-column transposed tile
 
 // Host-side launchers (sweep_kernels.hip).
 hipError_t launch_sweep(const GridView& g, int model, int axis, bool last, double dt_arg,

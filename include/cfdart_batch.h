@@ -5,10 +5,11 @@
  * (euler_equations/euler_simulation_code/sourcecode/main.cpp:17-49). Each grid in a batch
  * follows exactly the reference's time loop (simulation.hpp:48-61) with its own CFL time step,
  * step budget and end time; its results are bitwise those of the same grid run alone through
- * cfdart.h. Per step, one persistent kernel launch per sweep processes the tiles of every live
- * grid from a device work-list, and the step (sweep, sweep, loop bookkeeping) is captured once
- * in a HIP graph and replayed. Grids join and leave between steps by editing device-resident
- * tables, without re-capturing the graph.
+ * cfdart.h. Per sweep phase, one x-sweep and one y-sweep launch per model process the tiles of
+ * every live grid from device work-lists (one tile per workgroup; a grid's tiles for the axis it
+ * does not sweep in that phase exit at once), and the step (two phases, loop bookkeeping) is
+ * captured once in a HIP graph and replayed. Grids join and leave between steps by editing
+ * device-resident tables; the graph is re-captured only when a work-list outgrows its capacity.
  *
  * Splitting schemes: CFDART_NAIVE and CFDART_ALTERNATING (two sweeps per step).
  * Admissions, removals and tile changes take effect at the next cfdart_batch_step, which
@@ -36,8 +37,9 @@ void cfdart_batch_destroy(cfdart_batch* b);
 
 /* Admits a grid: uploads host_state (format of cfdart.h), computes its first dt and adds its
  * tiles to the work-list. It then runs until it has taken max_steps steps or reached t >= T.
- * y_strip_rows / x_rows_per_wave: tile shape as cfdart_set_tile, y strips only (0 = default:
- * 32-row strips, 8 rows per x wavefront). Returns CFDART_E_NOMEM when the
+ * y_strip_rows / x_rows_per_wave: tile shape. The y sweep of a batch runs the 16-column
+ * transposed tile: y_strip_rows is 0 or CFDART_Y_TRANSPOSED; x_rows_per_wave as cfdart_set_tile
+ * (0 = default 8). Returns CFDART_E_NOMEM when the
  * memory budget or the capacity would be exceeded (nothing is changed). slot_out receives the
  * grid's slot id. */
 int cfdart_batch_add(cfdart_batch* b, const cfdart_desc* desc, const double* host_state, int splitting,
@@ -67,7 +69,8 @@ int cfdart_batch_checksum(cfdart_batch* b, int slot, unsigned long long* out);
 /* Fallback counters of a grid (see CFDART_CNT_*), 4 values; synchronous. */
 int cfdart_batch_stats(cfdart_batch* b, int slot, long long* counters);
 int cfdart_batch_sync(cfdart_batch* b);
-/* Changes a grid's tile shape (used by the autotuner between steps); y strips only. */
+/* Changes a grid's tile shape (used by the autotuner between steps); y_strip_rows as in
+ * cfdart_batch_add. */
 int cfdart_batch_set_tile(cfdart_batch* b, int slot, int y_strip_rows, int x_rows_per_wave);
 /* Live grids, device bytes held by grid state, tiles in the work-list (each may be NULL; asking
  * for the tile count rebuilds a work-list left stale by admissions or removals). */

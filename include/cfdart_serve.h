@@ -59,7 +59,8 @@ typedef struct cfdart_serve_opts {
     int frame_every;         /* steps between frames of one grid; 0 = no frames */
     size_t frame_host_bytes; /* pinned host frame ring; frames overwrite it cyclically */
     int realtime;            /* 1: a request is visible from its arrival time on; 0: all at once */
-    int y_strip_rows[2];     /* batched tile shape per model (Euler, Allaire): y strips; 0 = default */
+    int y_strip_rows[2];     /* batched tile shape per model (Euler, Allaire), as cfdart_batch_add;
+                                0 = default */
     int x_rows_per_wave[2];
     int plain_y_strip_rows[2]; /* one-at-a-time tile shape per model, as cfdart_set_tile
                                   (transposed y tiles included); 0 = default */

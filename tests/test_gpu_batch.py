@@ -42,7 +42,8 @@ def test_batch_equals_alone(gpu, splitting):
     b = cfdart.Batch(16)
     slots = []
     for d, h, cfl, steps in cases:
-        slots.append(b.add(d, h, splitting=splitting, max_steps=steps, cfl=cfl, y_strip_rows=16, x_rows_per_wave=2))
+        slots.append(b.add(d, h, splitting=splitting, max_steps=steps, cfl=cfl, y_strip_rows=cfdart.Y_TRANSPOSED,
+                           x_rows_per_wave=2))
     b.step(30)
     b.sync()
     for (d, h, cfl, steps), s in zip(cases, slots):

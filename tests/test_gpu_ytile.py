@@ -94,12 +94,13 @@ def test_rejects_other_negative_rows(gpu):
     for yt in WIDTHS:
         g.set_tile(yt, 4)
     g.close()
-    # the batch's persistent kernel runs y strips only
+    # a batch runs the 16-column transposed y tile only
     b = cfdart.Batch(2)
-    for yt in WIDTHS:
+    for y in (cfdart.Y_TRANSPOSED_NARROW, 16):
         with pytest.raises(cfdart.CfdartError):
-            b.add(d, h, max_steps=2, y_strip_rows=yt)
-    s = b.add(d, h, max_steps=2)
+            b.add(d, h, max_steps=2, y_strip_rows=y)
+    s = b.add(d, h, max_steps=2, y_strip_rows=YT)
     with pytest.raises(cfdart.CfdartError):
-        b.set_tile(s, YT, 4)
+        b.set_tile(s, 32, 4)
+    b.set_tile(s, YT, 4)
     b.close()
