@@ -138,9 +138,9 @@ def bench_batched(args):
 
 def bench_served(args):
     """Served stream: Poisson arrivals (50/s, 70 % Euler, sides log-uniform in [64, 2048], steps
-    uniform in [50, 2000], a frame every 50 steps, 200 GB admission budget) replayed in real time
-    through the continuous-batching server, then the same requests one at a time with the plain
-    solver. The window is shorter than the full 120 s so that the run stays within minutes; the
+    uniform in [50, 2000], a frame every 50 steps, 200 GB admission budget, at most 32 M live
+    cells) replayed in real time through the continuous-batching server, then the same requests
+    one at a time with the plain solver. The window is shorter than the full 120 s so that the run stays within minutes; the
     offered load (about 31 G cell-updates/s) exceeds what one GPU serves, so requests queue."""
     import cfdart
     from cfdart import serve
@@ -149,49 +149,52 @@ def bench_served(args):
     work = float(sum(q.n * q.n * q.steps for q in reqs))
     log(f"served: {len(reqs)} requests over {args.serve_window:.1f} s, {work / 1e9:.1f} G cell-updates offered "
         f"({work / args.serve_window / 1e9:.1f} G/s)")
-    tile = {0: (cfdart.Y_TRANSPOSED, 8), 1: (cfdart.Y_TRANSPOSED, 4)}
+    tile = {0: (cfdart.Y_TRANSPOSED, 8), 1: (cfdart.Y_TRANSPOSED, 8)}
     plain = {0: (cfdart.Y_TRANSPOSED, 4), 1: (cfdart.Y_TRANSPOSED_NARROW, 2)}
-    o = serve.opts(mem_budget=200 * 10**9, capacity=8192, chunk_steps=10, frame_every=50,
-                   frame_host_bytes=2 << 30, realtime=True, tile=tile, plain_tile=plain)
+    # the server admits while at most 32 M cells are live (enough to fill the GPU); the rest wait
+    # first come first served. The same trace without the cap is reported beside it.
+    live_cap = 32 * 10**6
+
+    def opts(cap):
+        return serve.opts(mem_budget=200 * 10**9, capacity=8192, chunk_steps=10, frame_every=50,
+                          frame_host_bytes=2 << 30, realtime=True, tile=tile, plain_tile=plain, live_cells=cap)
+
+    o = opts(live_cap)
     rb, sb = serve.serve_batched(reqs, o)
     lb = serve.latency_stats(reqs, rb)
-    log(f"served batched: makespan {sb.makespan:.2f} s, {sb.cell_updates / sb.makespan / 1e9:.2f} G cell-updates/s, "
-        f"p50 {lb['p50']:.2f} s p99 {lb['p99']:.2f} s, max live {sb.max_live}, frames {sb.frames} "
-        f"({sb.frame_bytes / 1e9:.1f} GB)")
-    # the same trace with admission capped at 32 M live cells (enough to fill the GPU): the rest
-    # wait first come first served, which trades a little throughput for shorter latencies
-    live_cap = 32 * 10**6
-    oc = serve.opts(mem_budget=200 * 10**9, capacity=8192, chunk_steps=10, frame_every=50,
-                    frame_host_bytes=2 << 30, realtime=True, tile=tile, plain_tile=plain, live_cells=live_cap)
-    rc, sc = serve.serve_batched(reqs, oc)
-    lc = serve.latency_stats(reqs, rc)
-    log(f"served batched, at most {live_cap / 1e6:.0f} M live cells: makespan {sc.makespan:.2f} s, "
-        f"{sc.cell_updates / sc.makespan / 1e9:.2f} G cell-updates/s, p50 {lc['p50']:.2f} s p99 {lc['p99']:.2f} s, "
-        f"max live {sc.max_live}")
+    log(f"served batched, at most {live_cap / 1e6:.0f} M live cells: makespan {sb.makespan:.2f} s, "
+        f"{sb.cell_updates / sb.makespan / 1e9:.2f} G cell-updates/s, p50 {lb['p50']:.2f} s p99 {lb['p99']:.2f} s, "
+        f"max live {sb.max_live}, frames {sb.frames} ({sb.frame_bytes / 1e9:.1f} GB)")
+    ru, su = serve.serve_batched(reqs, opts(0))
+    lu = serve.latency_stats(reqs, ru)
+    log(f"served batched, no cap: makespan {su.makespan:.2f} s, {su.cell_updates / su.makespan / 1e9:.2f} G "
+        f"cell-updates/s, p50 {lu['p50']:.2f} s p99 {lu['p99']:.2f} s, max live {su.max_live}")
     ro, so = serve.serve_one_at_a_time(reqs, o)
     lo = serve.latency_stats(reqs, ro)
     busy = sum(r.service for r in ro)
     log(f"one at a time: makespan {so.makespan:.2f} s (busy {busy:.2f} s), "
         f"{so.cell_updates / so.makespan / 1e9:.2f} G cell-updates/s, p50 {lo['p50']:.2f} s p99 {lo['p99']:.2f} s")
-    same = sum(1 for a, b in zip(rb, ro) if a.status == 0 and a.checksum == b.checksum)
+
+    def same(ra, rb_):
+        return f"{sum(1 for a, b in zip(ra, rb_) if a.status == 0 and a.checksum == b.checksum)}/{len(reqs)}"
+
     return {
         "metric": "served aggregate cell-updates/s (FP64), continuous batching vs one-at-a-time",
         "value": sb.cell_updates / sb.makespan, "unit": "cell-updates/s", "n_gpus": 1,
         "higher_is_better": True, "dtype": "f64", "data": "synthetic (seeded Poisson trace, seeded disc/bubble ICs)",
         "config": {"workload": "served", "requests": len(reqs), "rate_per_s": 50.0, "window_s": args.serve_window,
                    "euler_frac": 0.7, "sides": "log-uniform [64,2048]", "steps": "uniform [50,2000]",
-                   "frame_every": 50, "mem_budget_bytes": 200 * 10**9, "chunk_steps": 10,
+                   "frame_every": 50, "mem_budget_bytes": 200 * 10**9, "live_cells": live_cap, "chunk_steps": 10,
                    "tile": {"euler": tile[0], "allaire": tile[1]}},
         "latency_s": {"batched": lb, "one_at_a_time": lo},
-        "batched_live_cap": {"live_cells": live_cap, "value": sc.cell_updates / sc.makespan, "makespan_s": sc.makespan,
-                             "latency_s": lc, "max_live": sc.max_live,
-                             "bitwise_equal_alone": f"{sum(1 for a, b in zip(rc, rb) if a.status == 0 and a.checksum == b.checksum)}/{len(reqs)}"},
         "one_at_a_time": {"value": so.cell_updates / so.makespan, "makespan_s": so.makespan, "busy_s": busy},
         "makespan_s": sb.makespan,
         "speedup_vs_one_at_a_time": (sb.cell_updates / sb.makespan) / (so.cell_updates / so.makespan),
         "max_live": sb.max_live, "peak_state_bytes": sb.peak_state_bytes,
         "frames": sb.frames, "frame_bytes": sb.frame_bytes,
-        "bitwise_equal_alone": f"{same}/{len(reqs)}",
+        "bitwise_equal_alone": same(rb, ro),
+        "no_live_cap": {"value": su.cell_updates / su.makespan, "makespan_s": su.makespan, "latency_s": lu,
+                        "max_live": su.max_live, "bitwise_equal_alone": same(ru, ro)},
     }
 
 
