@@ -34,6 +34,50 @@ namespace cfd {
 
 enum { EULER = 0, ALLAIRE = 1 };
 
+// FP64 division and square root, correctly rounded. On the device these are the steps of the
+// compiler's IEEE sequences without their operand scaling: v_rcp_f64 refined by two Newton
+// steps, the product, one residual correction and v_div_fixup for a / b; v_rsq_f64 refined by
+// Goldschmidt steps for sqrt. The scaling only acts on operands the solver's states never reach
+// (a divide whose numerator is below 2^-969 in magnitude, a subnormal divisor or quotient, an
+// exponent gap of 768 or more; a square root of a number below 2^-767), so every quotient and
+// root equals the IEEE one; v_div_fixup and the class test keep zeros, infinities and NaNs exact.
+CFD_HD double ddiv(double a, double b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-b, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    double q = a * r;
+    const double t = __builtin_fma(-b, q, a);
+    q = __builtin_fma(t, r, q);
+    return __builtin_amdgcn_div_fixup(q, b, a);
+#else
+    return a / b;
+#endif
+}
+
+CFD_HD double dsqrt(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;    // sqrt(x)
+    double h = y * 0.5;  // 1 / (2 sqrt(x))
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    // +0, -0 and +inf are their own roots (class mask: -0, +0, +inf)
+    return __builtin_amdgcn_class(x, 0x260) ? x : g;
+#else
+    return sqrt(x);
+#endif
+}
+
 template <int M> struct Model;
 template <> struct Model<EULER> {
     static constexpr int NC = 4;
@@ -85,8 +129,8 @@ struct Mix { double eps, k; };
 CFD_HD Mix mixture(const Material& m, double z)
 {
     Mix r;
-    r.eps = z / m.g1m1 + (1.0 - z) / m.g2m1;
-    r.k = (z * m.g1 * m.pinf1 / m.g1m1 + (1.0 - z) * m.g2 * m.pinf2 / m.g2m1) / r.eps;
+    r.eps = ddiv(z, m.g1m1) + ddiv(1.0 - z, m.g2m1);
+    r.k = ddiv(ddiv(z * m.g1 * m.pinf1, m.g1m1) + ddiv((1.0 - z) * m.g2 * m.pinf2, m.g2m1), r.eps);
     return r;
 }
 
@@ -94,18 +138,18 @@ template <int M> struct Eos;
 template <> struct Eos<EULER> {
     // E/headers/stiffened_gas_eos.hpp:57-70
     CFD_HD static double pressure(const Material& m, double rho, double e) { return m.g1m1 * rho * e - m.g1pinf1; }
-    CFD_HD static double specific_ie(const Material& m, double rho, double p) { return (p + m.g1pinf1) / (m.g1m1 * rho); }
-    CFD_HD static double soundspeed(const Material& m, double rho, double p) { return sqrt((m.g1 * (p + m.pinf1)) / rho); }
+    CFD_HD static double specific_ie(const Material& m, double rho, double p) { return ddiv(p + m.g1pinf1, m.g1m1 * rho); }
+    CFD_HD static double soundspeed(const Material& m, double rho, double p) { return dsqrt(ddiv(m.g1 * (p + m.pinf1), rho)); }
 };
 struct MixEos {
     // A/headers/mixturemodel.hpp:18-52
-    CFD_HD static double pressure(const Mix& x, double rho, double e) { return rho * e / x.eps - x.k; }
-    CFD_HD static double specific_ie(const Mix& x, double rho, double p) { return (p + x.k) * x.eps / rho; }
-    CFD_HD static double soundspeed(const Mix& x, double rho, double p) { return sqrt((x.k + p * ((1.0 / x.eps) + 1.0)) / rho); }
+    CFD_HD static double pressure(const Mix& x, double rho, double e) { return ddiv(rho * e, x.eps) - x.k; }
+    CFD_HD static double specific_ie(const Mix& x, double rho, double p) { return ddiv((p + x.k) * x.eps, rho); }
+    CFD_HD static double soundspeed(const Mix& x, double rho, double p) { return dsqrt(ddiv(x.k + p * (ddiv(1.0, x.eps) + 1.0), rho)); }
     CFD_HD static double shock_jump(const Mix& x, double p, double rho, double pstar)
     {
         double p_avg = 0.5 * (pstar + p);
-        return sqrt((pstar + x.k + p_avg / x.eps) / rho);
+        return dsqrt(ddiv(pstar + x.k + ddiv(p_avg, x.eps), rho));
     }
 };
 
@@ -121,9 +165,9 @@ template <> struct Prim<EULER> {
     CFD_HD void from_cons(const Material& m, const double* U)
     {
         double rho = U[0];
-        double u = U[1] / rho;
-        double v = U[2] / rho;
-        double e = U[3] / rho - 0.5 * (u * u + v * v);
+        double u = ddiv(U[1], rho);
+        double v = ddiv(U[2], rho);
+        double e = ddiv(U[3], rho) - 0.5 * (u * u + v * v);
         double p = Eos<EULER>::pressure(m, rho, e);
         W[0] = rho; W[1] = u; W[2] = v; W[3] = p;
         c = Eos<EULER>::soundspeed(m, rho, p);
@@ -135,9 +179,9 @@ template <> struct Prim<ALLAIRE> {
     CFD_HD void from_cons(const Material& m, const double* U)
     {
         double rho = U[0] + U[1];
-        double u = U[2] / rho;
-        double v = U[3] / rho;
-        double e = U[4] / rho - 0.5 * (u * u + v * v);
+        double u = ddiv(U[2], rho);
+        double v = ddiv(U[3], rho);
+        double e = ddiv(U[4], rho) - 0.5 * (u * u + v * v);
         Mix x = mixture(m, U[5]);
         double p = MixEos::pressure(x, rho, e);
         W[0] = U[0]; W[1] = U[1]; W[2] = u; W[3] = v; W[4] = p; W[5] = U[5];
@@ -186,17 +230,17 @@ CFD_HD void snap(double* U)
 CFD_HD bool is_physical(const double* U, Model<EULER>*)
 {
     double rho = U[0];
-    double u = U[1] / rho;
-    double v = U[2] / rho;
-    double e = U[3] / rho - 0.5 * (u * u + v * v);
+    double u = ddiv(U[1], rho);
+    double v = ddiv(U[2], rho);
+    double e = ddiv(U[3], rho) - 0.5 * (u * u + v * v);
     return rho >= 0.0 && e >= 0.0;
 }
 CFD_HD bool is_physical(const double* U, Model<ALLAIRE>*)
 {
     double rho = U[0] + U[1];
-    double u = U[2] / rho;
-    double v = U[3] / rho;
-    double e = U[4] / rho - 0.5 * (u * u + v * v);
+    double u = ddiv(U[2], rho);
+    double v = ddiv(U[3], rho);
+    double e = ddiv(U[4], rho) - 0.5 * (u * u + v * v);
     double z = U[5];
     return U[0] >= 0.0 && U[1] >= 0.0 && e >= 0.0 && z >= 0.0 && z <= 1.0;
 }
@@ -248,7 +292,7 @@ CFD_HD bool allaire_physical_screened(const double* U)
 
 CFD_HD double limit(int limiter, double a, double b)
 {
-    if (limiter == 1) return (a * b > 0.0) ? (2.0 * a * b) / (a + b) : 0.0;
+    if (limiter == 1) return (a * b > 0.0) ? ddiv(2.0 * a * b, a + b) : 0.0;
     if (b > 0.0) return smax(0.0, smin(a, b));
     return smin(0.0, smax(a, b));
 }
@@ -272,9 +316,9 @@ CFD_HD void side_from_cons(const Material& m, Side<EULER>& s)
 {
     const double* U = s.U;
     double rho = U[0];
-    double u = U[1] / rho;
-    double v = U[2] / rho;
-    double e = U[3] / rho - 0.5 * (u * u + v * v);
+    double u = ddiv(U[1], rho);
+    double v = ddiv(U[2], rho);
+    double e = ddiv(U[3], rho) - 0.5 * (u * u + v * v);
     double p = Eos<EULER>::pressure(m, rho, e);
     s.rho = rho; s.u = u; s.p = p;
     s.c = Eos<EULER>::soundspeed(m, rho, p);
@@ -284,9 +328,9 @@ CFD_HD void side_from_cons(const Material& m, Side<ALLAIRE>& s)
 {
     const double* U = s.U;
     double rho = U[0] + U[1];
-    double u = U[2] / rho;
-    double v = U[3] / rho;
-    double e = U[4] / rho - 0.5 * (u * u + v * v);
+    double u = ddiv(U[2], rho);
+    double v = ddiv(U[3], rho);
+    double e = ddiv(U[4], rho) - 0.5 * (u * u + v * v);
     double z = U[5];
     Mix x = mixture(m, z);
     double p = MixEos::pressure(x, rho, e);
@@ -301,9 +345,9 @@ CFD_HD void side_from_cons_mix(const Mix& x, Side<ALLAIRE>& s)
 {
     const double* U = s.U;
     double rho = U[0] + U[1];
-    double u = U[2] / rho;
-    double v = U[3] / rho;
-    double e = U[4] / rho - 0.5 * (u * u + v * v);
+    double u = ddiv(U[2], rho);
+    double v = ddiv(U[3], rho);
+    double e = ddiv(U[4], rho) - 0.5 * (u * u + v * v);
     double z = U[5];
     double p = MixEos::pressure(x, rho, e);
     s.rho = rho; s.u = u; s.p = p;
@@ -364,7 +408,7 @@ CFD_HD void jac_times(const double* W, double c, double s, const double* d, doub
 {
     double rho = W[0], u = W[1];
     double a00 = s * u, a01 = s * rho, z0 = s * 0.0;
-    double a13 = s * (1.0 / rho);
+    double a13 = s * ddiv(1.0, rho);
     double a31 = s * (rho * c * c);
     tmp[0] = z0 * d[3] + (z0 * d[2] + (a01 * d[1] + a00 * d[0]));
     tmp[1] = a13 * d[3] + (z0 * d[2] + (a00 * d[1] + z0 * d[0]));
@@ -376,7 +420,7 @@ CFD_HD void jac_times(const double* W, double c, double s, const double* d, doub
     double rho = W[0] + W[1];
     double su = s * W[2], z0 = s * 0.0;
     double a02 = s * W[0], a12 = s * W[1];
-    double a24 = s * (1.0 / rho);
+    double a24 = s * ddiv(1.0, rho);
     double a42 = s * (rho * c * c);
     tmp[0] = z0 * d[5] + (z0 * d[4] + (z0 * d[3] + (a02 * d[2] + (z0 * d[1] + su * d[0]))));
     tmp[1] = z0 * d[5] + (z0 * d[4] + (z0 * d[3] + (a12 * d[2] + (su * d[1] + z0 * d[0]))));
@@ -398,7 +442,7 @@ CFD_HD bool extrap_physical(const Material& m, const double* T, Model<ALLAIRE>*)
 {
     // no snapping can occur when both partial densities and z are away from 0 (and z from 1)
     if (T[0] >= 1e-12 && T[1] >= 1e-12 && T[5] >= 1e-12 && T[5] <= 1.0 - 1e-12 &&
-        prim_clearly_physical(T[0] + T[1], T[2], T[3], T[4], m.kmin, 1.0 / m.epsmin))
+        prim_clearly_physical(T[0] + T[1], T[2], T[3], T[4], m.kmin, ddiv(1.0, m.epsmin)))
         return true;
     double U[6];
     p2c<ALLAIRE>(m, T, U);
@@ -476,17 +520,17 @@ CFD_HD void hllc(const Material& m, int wave_speeds, const Side<M>& L, const Sid
         SL = smin(u_avg - c_avg, uL - cL);
         SR = smax(u_avg + c_avg, uR + cR);
     }
-    double us = (pR - pL + rhoL * uL * (SL - uL) - rhoR * uR * (SR - uR))
-                / (rhoL * (SL - uL) - rhoR * (SR - uR));
-    double fL = (SL - uL) / (SL - us);
-    double fR = (SR - uR) / (SR - us);
+    double us = ddiv(pR - pL + rhoL * uL * (SL - uL) - rhoR * uR * (SR - uR),
+                     rhoL * (SL - uL) - rhoR * (SR - uR));
+    double fL = ddiv(SL - uL, SL - us);
+    double fR = ddiv(SR - uR, SR - us);
     double sg = copysign(1.0, us);
     double wL = (1.0 + sg) / 2.0;
     double wR = (1.0 - sg) / 2.0;
     double mSL = smin(0.0, SL);
     double MSR = smax(0.0, SR);
-    double eL = L.U[NC == 4 ? 3 : 4] + (us - uL) * (rhoL * us + (pL / (SL - uL)));
-    double eR = R.U[NC == 4 ? 3 : 4] + (us - uR) * (rhoR * us + (pR / (SR - uR)));
+    double eL = L.U[NC == 4 ? 3 : 4] + (us - uL) * (rhoL * us + ddiv(pL, SL - uL));
+    double eR = R.U[NC == 4 ? 3 : 4] + (us - uR) * (rhoR * us + ddiv(pR, SR - uR));
     for (int k = 0; k < NC; ++k) {
         double usl, usr, fl, fr;
         if (M == ALLAIRE && k == 5) {
@@ -548,14 +592,14 @@ CFD_HD bool euler_repair(const Material& m, const double* old, double* fut)
 {
     if (cons_clearly_physical(fut[0], fut[1], fut[2], fut[3])) return false;
     double rho = fut[0];
-    double u = fut[1] / rho;
-    double v = fut[2] / rho;
-    double e = fut[3] / rho - 0.5 * (u * u + v * v);
+    double u = ddiv(fut[1], rho);
+    double v = ddiv(fut[2], rho);
+    double e = ddiv(fut[3], rho) - 0.5 * (u * u + v * v);
     if (rho >= 0.0 && e >= 0.0) return false;
     double rho_old = old[0];
-    double u_old = old[1] / rho_old;
-    double v_old = old[2] / rho_old;
-    double e_old = old[3] / rho_old - 0.5 * (u_old * u_old + v_old * v_old);
+    double u_old = ddiv(old[1], rho_old);
+    double v_old = ddiv(old[2], rho_old);
+    double e_old = ddiv(old[3], rho_old) - 0.5 * (u_old * u_old + v_old * v_old);
     if (rho <= 1e-12) rho = rho_old;
     if (e <= 1e-12) e = e_old;
     double p = Eos<EULER>::pressure(m, rho, e);
@@ -596,9 +640,9 @@ CFD_HD void godunov_face(const Material& m, int wave_speeds, const double* Ua, c
 CFD_HD void wave_speeds(const Material& m, const double* U, double& su, double& sv, Model<EULER>*)
 {
     double rho = U[0];
-    double u = U[1] / rho;
-    double v = U[2] / rho;
-    double e = U[3] / rho - 0.5 * (u * u + v * v);
+    double u = ddiv(U[1], rho);
+    double v = ddiv(U[2], rho);
+    double e = ddiv(U[3], rho) - 0.5 * (u * u + v * v);
     double p = Eos<EULER>::pressure(m, rho, e);
     double c = Eos<EULER>::soundspeed(m, rho, p);
     su = fabs(u) + c;
@@ -607,9 +651,9 @@ CFD_HD void wave_speeds(const Material& m, const double* U, double& su, double& 
 CFD_HD void wave_speeds(const Material& m, const double* U, double& su, double& sv, Model<ALLAIRE>*)
 {
     double rho = U[0] + U[1];
-    double u = U[2] / rho;
-    double v = U[3] / rho;
-    double e = U[4] / rho - 0.5 * (u * u + v * v);
+    double u = ddiv(U[2], rho);
+    double v = ddiv(U[3], rho);
+    double e = ddiv(U[4], rho) - 0.5 * (u * u + v * v);
     double z = U[5];
     Mix x = mixture(m, z);
     double p = MixEos::pressure(x, rho, e);
