@@ -651,6 +651,21 @@ __device__ __forceinline__ void fold_maxima(Ctrl* c, double& mu, double& mv)
     mv = __longlong_as_double((long long)b);
 }
 
+// The same fold by one whole wavefront, lane i taking slot i: every lane of the calling
+// wavefront receives the maxima. The slots hold non-negative doubles, whose bit patterns order
+// as their values, so the floating-point maximum equals the serial fold's.
+__device__ __forceinline__ void fold_maxima_wave(Ctrl* c, double& mu, double& mv)
+{
+    static_assert(MAX_SLOTS == 64, "one slot per lane");
+    const int lane = threadIdx.x & 63;
+    const double a = __longlong_as_double((long long)c->maxu[lane]);
+    const double b = __longlong_as_double((long long)c->maxv[lane]);
+    c->maxu[lane] = 0;
+    c->maxv[lane] = 0;
+    mu = wave_max(a);
+    mv = wave_max(b);
+}
+
 // dt from the folded maxima (E/sourcecode/euler_timestep.cpp:35-43).
 __device__ __forceinline__ double dt_from_maxima(const Ctrl* c, double mu, double mv, int n, double t, double dx,
                                                  double dy)

@@ -172,29 +172,29 @@ __global__ void mass_record_kernel(const Ctrl* c, const double* partials, int nb
 }
 
 // Start of the time loop: dt for (n, t) from a fresh reduction, and the 'while (t < T)' test.
-__global__ void init_dt_kernel(Ctrl* c, double dx, double dy)
+__global__ __launch_bounds__(64) void init_dt_kernel(Ctrl* c, double dx, double dy)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
     double mu, mv;
-    fold_maxima(c, mu, mv);
+    fold_maxima_wave(c, mu, mv);  // the whole wavefront
+    if (threadIdx.x != 0) return;
     c->active = c->t < c->T;
     if (c->active) c->dt = dt_from_maxima(c, mu, mv, c->n, c->t, dx, dy);
 }
 
 // End of a step (E/headers/simulation.hpp:68-70): t += dt, n++, snap to T, loop test, and the
 // next step's dt from the maxima that the step's last sweep accumulated.
-__global__ void finalize_step_kernel(Ctrl* c, double dx, double dy)
+__global__ __launch_bounds__(64) void finalize_step_kernel(Ctrl* c, double dx, double dy)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (!c->active) return;
+    if (!c->active) return;  // the same value in every lane
+    double mu, mv;
+    fold_maxima_wave(c, mu, mv);  // the whole wavefront
+    if (threadIdx.x != 0) return;
     double t = c->t + c->dt;
     int n = c->n + 1;
     if (fabs(t - c->T) < 1e-12) t = c->T;
     c->t = t;
     c->n = n;
     c->active = t < c->T;
-    double mu, mv;
-    fold_maxima(c, mu, mv);
     if (c->active) c->dt = dt_from_maxima(c, mu, mv, n, t, dx, dy);
 }
 
