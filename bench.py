@@ -44,24 +44,49 @@ def read_traffic(workload, kernel):
     return None
 
 
-def cpu_baseline(steps):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(steps, steps_1t):
+    """The oracle on the host's cores over the first `steps` steps of euler_2048, and on one
+    thread over the first `steps_1t` steps (a scalar reference point)."""
     import cfdart
     import oracle
     d, h = cfdart.configs.euler_2048()
+    cells = d.nx * d.ny
+    L = oracle.lib()
+    cores = L.orc_get_threads()
     sim = oracle.Sim(d, h)
     t0 = time.perf_counter()
     k, _ = sim.run(cfdart.ALTERNATING, steps, cfl=0.8)
     el = time.perf_counter() - t0
-    cells = d.nx * d.ny
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {
+    out = {
         "value": cells * k / el,
         "unit": "cell-updates/s",
         "cores": cores,
         "kind": "port",
         "sample": f"oracle (FP64 C restatement of the reference), first {k} steps of euler_2048 "
-                  f"from its initial condition, {el:.1f} s on {cores} OpenMP threads",
+                  f"from its initial condition, {el:.1f} s on {cores} OpenMP threads ({cpu_model()})",
     }
+    if steps_1t > 0:
+        prev = L.orc_set_threads(1)
+        try:
+            sim1 = oracle.Sim(d, h)
+            t0 = time.perf_counter()
+            k1, _ = sim1.run(cfdart.ALTERNATING, steps_1t, cfl=0.8)
+            el1 = time.perf_counter() - t0
+        finally:
+            L.orc_set_threads(prev)
+        out["one_thread"] = {"value": cells * k1 / el1, "unit": "cell-updates/s", "cores": 1,
+                             "sample": f"first {k1} steps, {el1:.1f} s"}
+    return out
 
 
 def bench_batched(args):
@@ -205,6 +230,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-steps", type=int, default=160,
                     help="oracle steps of euler_2048 for the CPU baseline, about 10 s on 16 threads (0: skip)")
+    ap.add_argument("--cpu-steps-1t", type=int, default=8,
+                    help="oracle steps of euler_2048 on one thread, about 10 s (0: skip)")
     ap.add_argument("--strip-rows", type=int, default=0)
     ap.add_argument("--tile", default="", help="Y_STRIP_ROWS,X_ROWS_PER_WAVE: fixed tile shape, no autotuning")
     ap.add_argument("--workload", default="euler_2048", choices=["euler_2048", "allaire_1024", "batched", "served"])
@@ -314,7 +341,7 @@ def main():
         "fallback_counters": counters,
     }
     if args.cpu_steps > 0 and args.workload == "euler_2048":
-        out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_steps, args.cpu_steps_1t)
     print(json.dumps(out), flush=True)
 
 

@@ -743,3 +743,12 @@ int orc_run(const orc_desc* d, int splitting, double* grid, double* future, int 
     *t_inout = t;
     return k;
 }
+
+int orc_set_threads(int n)
+{
+    const int prev = omp_get_max_threads();
+    if (n > 0) omp_set_num_threads(n);
+    return prev;
+}
+
+int orc_get_threads(void) { return omp_get_max_threads(); }

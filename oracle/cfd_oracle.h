@@ -96,6 +96,10 @@ double orc_specific_ie(const orc_desc* d, double rho, double p, double z);
 double orc_soundspeed(const orc_desc* d, double rho, double p, double z);
 double orc_shock_speed_jump(const orc_desc* d, double p, double rho, double pstar, double z);
 
+/* OpenMP threads of the sweeps: set (n > 0) and return the previous maximum; query. */
+int orc_set_threads(int n);
+int orc_get_threads(void);
+
 #ifdef __cplusplus
 }
 #endif

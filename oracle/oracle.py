@@ -55,6 +55,10 @@ def lib():
                                   C.c_double, C.c_double, C.c_double, C.c_double]
         L.orc_zupdate.restype = C.c_double
         L.orc_mixture_eos.argtypes = [D, C.c_double, dp, dp]
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_set_threads.restype = C.c_int
+        L.orc_get_threads.argtypes = []
+        L.orc_get_threads.restype = C.c_int
         for f in ("orc_pressure", "orc_specific_ie", "orc_soundspeed"):
             getattr(L, f).argtypes = [D, C.c_double, C.c_double, C.c_double]
             getattr(L, f).restype = C.c_double
