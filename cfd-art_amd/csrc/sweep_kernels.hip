@@ -4,8 +4,10 @@
 // x sweep: one cell per lane along x (coalesced 512-byte rows per component), a wavefront
 //   covers 64 consecutive columns of which the middle 60 are written; neighbour values move
 //   between lanes with DPP wave shifts, so every per-cell quantity is computed once.
-// y sweep: one column per lane; each lane marches down a strip of rows keeping a three-cell
-//   register window (loads are coalesced across the wavefront, no cross-lane traffic).
+// y sweep, transposed tile (the default for MUSCL-Hancock): a workgroup stages 64 rows x 16 (or
+//   8) columns through LDS and each wavefront computes columns as the x sweep computes rows,
+//   lanes along y with DPP neighbours; strip tile (the Godunov scheme, or on request): one
+//   column per lane, each lane marching down a strip of rows with a three-cell register window.
 // Both fuse the ghost fill (mirrored indexing) of E/sourcecode/euler_ppsweep.cpp:7-118, the
 // face fluxes, the conservative update, the Euler repair / Allaire z-update and snapping, and,
 // in the last sweep of a step, the per-cell wave speeds of compute_dt
