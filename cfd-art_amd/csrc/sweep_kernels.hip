@@ -11,7 +11,7 @@
 // Both fuse the ghost fill (mirrored indexing) of E/sourcecode/euler_ppsweep.cpp:7-118, the
 // face fluxes, the conservative update, the Euler repair / Allaire z-update and snapping, and,
 // in the last sweep of a step, the per-cell wave speeds of compute_dt
-// (E/sourcecode/euler_timestep.cpp:14-25) reduced to one atomic max per wavefront.
+// (E/sourcecode/euler_timestep.cpp:14-25) reduced to one atomic max per workgroup.
 #include "sweep_device.h"
 
 #include <hip/hip_runtime.h>
