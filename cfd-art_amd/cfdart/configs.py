@@ -5,6 +5,7 @@ euler_200      Euler, gamma 1.4, 200x200 on [0,1]^2, ambient (1,0,0,1), 8 seeded
                van Leer, HLLC, alternating splitting, CFL 0.8, run to t = 0.2.
 euler_2048     As above at 2048x2048 with 64 discs of r U[0.005,0.03]; reflective left/right,
                transmissive bottom/top; 100 steps at CFL 0.8.
+euler_slab     One 2048x256 y-slab of euler_2048 (one GPU's share of eight).
 allaire_1024   Allaire water (gamma 4.4, pinf 6e8, rho 1000) with 16 air bubbles (gamma 1.4,
                rho 1), p 1e5, radii U[0.02,0.06], centres U[0.15,0.95]x[0.05,0.95]; a planar
                shock at x = 0.1 (post-shock water p 1e9, rho 1323.65, u 681.58); z clipped to
@@ -40,6 +41,18 @@ def euler_200(seed=1):
 
 def euler_2048(seed=1):
     return euler_discs(2048, 64, (0.005, 0.03), (1, 1, 0, 0), seed)
+
+
+def euler_slab(rows=256, index=3, seed=1):
+    """One GPU's share of euler_2048 split into 2048/rows y-slabs of full rows: rows
+    [index*rows, (index+1)*rows) of its initial state, as a 2048 x rows grid with the same cell
+    size. Its bottom and top sides are transmissive here; in a sharded run they would read the
+    neighbouring slabs' halo rows instead, which costs the sweeps the same work."""
+    d, h = euler_2048(seed)
+    ng = d.ng
+    ds = L.make_desc(L.EULER, d.nx, rows, lx=1.0, ly=rows / d.nx, gamma1=1.4, bc=(1, 1, 0, 0), limiter=d.limiter)
+    y0 = index * rows
+    return ds, h[y0:y0 + rows + 2 * ng].copy()
 
 
 def allaire_bubbles(n=1024, n_bubbles=16, seed=1, ny=None, wave_speeds=L.PVRS, limiter=L.MINMOD):
