@@ -54,33 +54,34 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(steps, steps_1t):
-    """The oracle on the host's cores over the first `steps` steps of euler_2048, and on one
-    thread over the first `steps_1t` steps (a scalar reference point)."""
+def cpu_baseline(cfg, cfl, steps, steps_1t, T=math.inf, label="euler_2048"):
+    """The oracle on the host's cores over the first `steps` steps of a workload (to T when T is
+    finite), and on one thread over its first `steps_1t` steps (a scalar reference point)."""
     import cfdart
     import oracle
-    d, h = cfdart.configs.euler_2048()
+    d, h = cfg()
     cells = d.nx * d.ny
     L = oracle.lib()
     cores = L.orc_get_threads()
     sim = oracle.Sim(d, h)
     t0 = time.perf_counter()
-    k, _ = sim.run(cfdart.ALTERNATING, steps, cfl=0.8)
+    k, _ = sim.run(cfdart.ALTERNATING, steps, T=T, cfl=cfl)
     el = time.perf_counter() - t0
+    span = f"all {k} steps to t = {T}" if math.isfinite(T) else f"first {k} steps"
     out = {
         "value": cells * k / el,
         "unit": "cell-updates/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"oracle (FP64 C restatement of the reference), first {k} steps of euler_2048 "
-                  f"from its initial condition, {el:.1f} s on {cores} OpenMP threads ({cpu_model()})",
+        "sample": f"oracle (FP64 C restatement of the reference), {span} of {label} from its initial condition, "
+                  f"{el:.1f} s on {cores} OpenMP threads ({cpu_model()})",
     }
     if steps_1t > 0:
         prev = L.orc_set_threads(1)
         try:
             sim1 = oracle.Sim(d, h)
             t0 = time.perf_counter()
-            k1, _ = sim1.run(cfdart.ALTERNATING, steps_1t, cfl=0.8)
+            k1, _ = sim1.run(cfdart.ALTERNATING, steps_1t, T=T, cfl=cfl)
             el1 = time.perf_counter() - t0
         finally:
             L.orc_set_threads(prev)
@@ -235,7 +236,7 @@ def main():
     ap.add_argument("--strip-rows", type=int, default=0)
     ap.add_argument("--tile", default="", help="Y_STRIP_ROWS,X_ROWS_PER_WAVE: fixed tile shape, no autotuning")
     ap.add_argument("--workload", default="euler_2048",
-                    choices=["euler_2048", "allaire_1024", "euler_slab", "batched", "served"])
+                    choices=["euler_2048", "allaire_1024", "euler_200", "euler_slab", "batched", "served"])
     ap.add_argument("--serve-window", type=float, default=10.0, help="seconds of the arrival trace to replay")
     ap.add_argument("--grids", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=500)
@@ -273,7 +274,15 @@ def main():
                            metric="cell-updates/s (2048^2 FP64 Euler, MUSCL-Hancock + HLLC)",
                            data="synthetic (seeded disc generator)",
                            config={"discs": 64, "bc": "reflective L/R, transmissive B/T", "limiter": "van Leer",
-                                   "riemann": "HLLC"}),
+                                   "riemann": "HLLC"},
+                           cpu=dict(steps=args.cpu_steps, steps_1t=args.cpu_steps_1t)),
+        "euler_200": dict(cfg=cfdart.configs.euler_200, model=cfdart.EULER, cfl=0.8, nc=4, tag="EULER", T=0.2,
+                          metric="cell-updates/s (200^2 FP64 Euler to t = 0.2, MUSCL-Hancock + HLLC; the "
+                                 "CPU-runnable case)",
+                          data="synthetic (seeded disc generator)",
+                          config={"discs": 8, "bc": "transmissive", "limiter": "van Leer", "riemann": "HLLC",
+                                  "T": 0.2},
+                          cpu=dict(steps=100000, steps_1t=100000)),
         "euler_slab": dict(cfg=cfdart.configs.euler_slab, model=cfdart.EULER, cfl=0.8, nc=4, tag="EULER",
                            metric="cell-updates/s per GPU (2048x256 y-slab of 2048^2 FP64 Euler: one GPU's share of 8)",
                            data="synthetic (rows 768-1023 of the seeded 2048^2 disc state)",
@@ -283,8 +292,12 @@ def main():
                              metric="cell-updates/s (1024^2 FP64 Allaire five-equation, MUSCL-Hancock + PVRS-HLLC)",
                              data="synthetic (seeded bubble generator, planar shock)",
                              config={"bubbles": 16, "bc": "transmissive", "limiter": "minmod",
-                                     "riemann": "PVRS-HLLC", "z_clip": [1e-6, 1 - 1e-6]}),
+                                     "riemann": "PVRS-HLLC", "z_clip": [1e-6, 1 - 1e-6]},
+                             cpu=dict(steps=100, steps_1t=4)),
     }[args.workload]
+    T_end = single.get("T", math.inf)
+    if math.isfinite(T_end):  # a run to T: every step of it is timed, no warm-up steps
+        args.warmup = 0
     d, h = single["cfg"]()
     cfl = single["cfl"]
     cells = d.nx * d.ny
@@ -298,7 +311,8 @@ def main():
     if args.strip_rows:
         g.set_strip_rows(args.strip_rows)
     g.upload(h)
-    log(f"{args.workload}: warm-up {args.warmup} steps, timed {args.steps} steps")
+    if not math.isfinite(T_end):
+        log(f"{args.workload}: warm-up {args.warmup} steps, timed {args.steps} steps")
     g.run(cfdart.ALTERNATING, args.warmup, cfl=cfl)
     # bring the GPU to its working clocks on a scratch copy of the same state, so that the timed
     # steps of the measured grid (steps warmup .. warmup+steps of the run) are not taken at idle
@@ -306,17 +320,25 @@ def main():
     scratch = cfdart.Grid(d)
     scratch.set_tile(tuned["y_strip_rows"], tuned["x_rows_per_wave"])
     scratch.upload(h)
-    scratch.run(cfdart.ALTERNATING, 100, cfl=cfl)
+    if math.isfinite(T_end):
+        # the scratch run to T also gives the number of steps the timed run to T takes
+        t_s, n_s = scratch.run(cfdart.ALTERNATING, 20000, T=T_end, cfl=cfl)
+        if t_s != T_end:
+            raise SystemExit(f"bench.py: {args.workload} did not reach t = {T_end} in 20000 steps")
+        args.steps = n_s
+        log(f"{args.workload}: the run to t = {T_end} takes {n_s} steps, all timed")
+    else:
+        scratch.run(cfdart.ALTERNATING, 100, cfl=cfl)
     scratch.close()
     g.set_profiling(True)
     g.sync()
     t0 = time.perf_counter()
-    g.run(cfdart.ALTERNATING, args.steps, cfl=cfl, sync=False)
+    g.run(cfdart.ALTERNATING, args.steps, T=T_end, cfl=cfl, sync=False)
     g.sync()
     el = time.perf_counter() - t0
     prof = g.profile()
     g.set_profiling(False)
-    t_sim, n_sim = g.run(cfdart.ALTERNATING, 0, cfl=cfl)
+    t_sim, n_sim = g.run(cfdart.ALTERNATING, 0, T=T_end, cfl=cfl)
     counters = g.stats()
 
     bytes_per_launch = 16.0 * single["nc"] * cells  # one FP64 read + write of the state per cell
@@ -354,8 +376,10 @@ def main():
         "tile": {"y_strip_rows": tuned["y_strip_rows"], "x_rows_per_wave": tuned["x_rows_per_wave"]},
         "fallback_counters": counters,
     }
-    if args.cpu_steps > 0 and args.workload == "euler_2048":
-        out["cpu_baseline"] = cpu_baseline(args.cpu_steps, args.cpu_steps_1t)
+    if args.cpu_steps > 0 and single.get("cpu"):
+        c = single["cpu"]
+        out["cpu_baseline"] = cpu_baseline(single["cfg"], cfl, c["steps"], c["steps_1t"], T=single.get("T", math.inf),
+                                           label=args.workload)
     print(json.dumps(out), flush=True)
 
 
