@@ -244,8 +244,8 @@ def main():
                     help="run only a plain device copy of the state size (PMC calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (the workload is a single 2048^2 grid); "
-              "run with --gpus 1", file=sys.stderr)
+        print("bench.py: measures one GPU only (the workloads run on one device; sharding across GPUs is not "
+              "implemented); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import cfdart
