@@ -1,10 +1,12 @@
 """GPU parity at the benchmark's full sizes.
 
-euler_2048 (2048x2048, 64 discs) after 100 alternating steps at CFL 0.8 and allaire_1024
-(1024x1024, 16 bubbles behind a planar shock) after a 20-step prefix, both against the oracle
-on the same seeded initial state: bitwise, and per-field relative L1/Linf within 1e-10. Also
-size-independent invariants on the device: conservation of mass and energy in a closed
-(all-reflective) 2048x2048 box over 100 steps, and free-stream preservation.
+euler_2048 (2048x2048, 64 discs) after 100 alternating steps at CFL 0.8, allaire_1024
+(1024x1024, 16 bubbles behind a planar shock) after a 100-step prefix and the 2048x256 slab of
+euler_2048 after 30 steps, all against the oracle on the same seeded initial state: bitwise,
+and per-field relative L1/Linf within 1e-10. Also size-independent checks on the device: the
+whole 2000-step allaire_1024 run agrees across tile shapes and keeps every state physical,
+mass and energy are conserved in a closed (all-reflective) 2048x2048 box over 100 steps, and a
+free stream is preserved.
 """
 import math
 import time
@@ -46,14 +48,49 @@ def test_euler_2048_100_steps(gpu):
 
 def test_allaire_1024_prefix(gpu):
     d, h = cfdart.configs.allaire_1024()
-    print("case allaire_1024, 20 alternating steps, CFL 0.5", flush=True)
+    print("case allaire_1024, 100 alternating steps, CFL 0.5", flush=True)
     g = cfdart.Grid(d)
     g.upload(h)
-    tg, ng = g.run(cfdart.ALTERNATING, 20, cfl=0.5)
+    tg, ng = g.run(cfdart.ALTERNATING, 100, cfl=0.5)
     ora = oracle.Sim(d, h)
-    no, _ = ora.run(cfdart.ALTERNATING, 20, cfl=0.5)
-    assert ng == no == 20 and tg == ora.t
-    _compare(g.download(), ora.grid, "allaire_1024 n=20")
+    no, _ = ora.run(cfdart.ALTERNATING, 100, cfl=0.5)
+    assert ng == no == 100 and tg == ora.t
+    _compare(g.download(), ora.grid, "allaire_1024 n=100")
+    assert g.stats() == ora.counters()
+
+
+def test_allaire_1024_full_run_tiles_agree(gpu):
+    """The whole 2000-step allaire_1024 run (too long for the oracle in a test): the transposed
+    8-column tile and 8-row strips end on the same bits with the same counters, no state fails
+    the physical-state test, and z stays inside its clip bounds."""
+    d, h = cfdart.configs.allaire_1024()
+    out = []
+    for tile in [(cfdart.Y_TRANSPOSED_NARROW, 2), (8, 4)]:
+        print(f"case allaire_1024, 2000 steps, tile {tile}", flush=True)
+        g = cfdart.Grid(d)
+        g.set_tile(*tile)
+        g.upload(h)
+        t, n = g.run(cfdart.ALTERNATING, 2000, cfl=0.5)
+        z = interior(g.download())[..., 5]
+        out.append((t, n, g.checksum(), g.stats()))
+        g.close()
+        assert n == 2000 and math.isfinite(t)
+        assert z.min() >= d.z_lo and z.max() <= d.z_hi
+    assert out[0] == out[1]
+    assert out[0][3][3] == 0  # no unphysical state
+
+
+def test_euler_slab_parity(gpu):
+    """One GPU's 2048x256 share of euler_2048 (the sharding measurement's grid), 30 steps."""
+    d, h = cfdart.configs.euler_slab()
+    print("case euler_slab 2048x256, 30 alternating steps", flush=True)
+    g = cfdart.Grid(d)
+    g.upload(h)
+    tg, ng = g.run(cfdart.ALTERNATING, 30, cfl=0.8)
+    ora = oracle.Sim(d, h)
+    no, _ = ora.run(cfdart.ALTERNATING, 30, cfl=0.8)
+    assert ng == no == 30 and tg == ora.t
+    _compare(g.download(), ora.grid, "euler_slab n=30")
     assert g.stats() == ora.counters()
 
 
