@@ -236,7 +236,8 @@ def main():
     ap.add_argument("--strip-rows", type=int, default=0)
     ap.add_argument("--tile", default="", help="Y_STRIP_ROWS,X_ROWS_PER_WAVE: fixed tile shape, no autotuning")
     ap.add_argument("--workload", default="euler_2048",
-                    choices=["euler_2048", "allaire_1024", "euler_200", "euler_slab", "batched", "served"])
+                    choices=["euler_2048", "allaire_1024", "euler_200", "euler_slab", "godunov_2048", "batched",
+                             "served"])
     ap.add_argument("--serve-window", type=float, default=10.0, help="seconds of the arrival trace to replay")
     ap.add_argument("--grids", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=500)
@@ -283,6 +284,11 @@ def main():
                           config={"discs": 8, "bc": "transmissive", "limiter": "van Leer", "riemann": "HLLC",
                                   "T": 0.2},
                           cpu=dict(steps=100000, steps_1t=100000)),
+        "godunov_2048": dict(cfg=cfdart.configs.euler_2048_godunov, model=cfdart.EULER, cfl=0.8, nc=4, tag="EULER",
+                             metric="cell-updates/s (2048^2 FP64 Euler, first-order Godunov + HLLC)",
+                             data="synthetic (seeded disc generator)",
+                             config={"discs": 64, "bc": "reflective L/R, transmissive B/T", "riemann": "HLLC",
+                                     "scheme": "Godunov (first order), y strips"}),
         "euler_slab": dict(cfg=cfdart.configs.euler_slab, model=cfdart.EULER, cfl=0.8, nc=4, tag="EULER",
                            metric="cell-updates/s per GPU (2048x256 y-slab of 2048^2 FP64 Euler: one GPU's share of 8)",
                            data="synthetic (rows 768-1023 of the seeded 2048^2 disc state)",
@@ -348,7 +354,8 @@ def main():
             per_kernel[axis] = {"avg_us": 1e3 * ms / nl, "launches": nl,
                                 "gbs": bytes_per_launch / (ms / nl * 1e-3) / 1e9}
     dom = max(per_kernel, key=lambda a: per_kernel[a]["avg_us"] * per_kernel[a]["launches"])
-    transposed = dom == "y" and tuned["y_strip_rows"] < 0 and not args.strip_rows
+    transposed = (dom == "y" and tuned["y_strip_rows"] < 0 and not args.strip_rows
+                  and d.flux_solver == cfdart.MUSCL_HANCOCK)  # the Godunov scheme runs y strips
     kname = f"sweep_{'yt' if transposed else dom}_kernel<{single['tag']}>"
     achieved = per_kernel[dom]["gbs"]
     traffic = read_traffic(args.workload, dom)

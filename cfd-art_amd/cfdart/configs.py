@@ -6,6 +6,7 @@ euler_200      Euler, gamma 1.4, 200x200 on [0,1]^2, ambient (1,0,0,1), 8 seeded
 euler_2048     As above at 2048x2048 with 64 discs of r U[0.005,0.03]; reflective left/right,
                transmissive bottom/top; 100 steps at CFL 0.8.
 euler_slab     One 2048x256 y-slab of euler_2048 (one GPU's share of eight).
+euler_2048_godunov  euler_2048 with the first-order Godunov flux.
 allaire_1024   Allaire water (gamma 4.4, pinf 6e8, rho 1000) with 16 air bubbles (gamma 1.4,
                rho 1), p 1e5, radii U[0.02,0.06], centres U[0.15,0.95]x[0.05,0.95]; a planar
                shock at x = 0.1 (post-shock water p 1e9, rho 1323.65, u 681.58); z clipped to
@@ -41,6 +42,14 @@ def euler_200(seed=1):
 
 def euler_2048(seed=1):
     return euler_discs(2048, 64, (0.005, 0.03), (1, 1, 0, 0), seed)
+
+
+def euler_2048_godunov(seed=1):
+    """euler_2048 with the first-order Godunov flux (ghost width 1, as the reference sets it)."""
+    d, p = euler_discs_params(2048, 64, (0.005, 0.03), (1, 1, 0, 0), seed)
+    dg = L.make_desc(L.EULER, d.nx, d.ny, lx=1.0, ly=1.0, gamma1=1.4, bc=(1, 1, 0, 0), limiter=d.limiter,
+                     flux_solver=L.GODUNOV)
+    return dg, L.random_discs(dg, p)
 
 
 def euler_slab(rows=256, index=3, seed=1):

@@ -128,7 +128,7 @@ int cfdart_set_strip_rows(cfdart_grid* g, int rows);
  * selects strips of that many rows, 256 columns wide, one column per lane marching down the
  * strip; y_strip_rows = CFDART_Y_TRANSPOSED (the default) selects the transposed tile: 64 rows x
  * 16 columns staged in LDS, lanes along y, 60 rows written; CFDART_Y_TRANSPOSED_NARROW the same
- * with 8 columns. The Godunov scheme has no transposed tile and runs 64-row strips in its place.
+ * with 8 columns. The Godunov scheme has no transposed tile and runs 8-row strips in its place.
  * Every shape gives the same bits. */
 #define CFDART_Y_TRANSPOSED (-1)
 #define CFDART_Y_TRANSPOSED_NARROW (-2)
