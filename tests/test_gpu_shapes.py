@@ -71,3 +71,23 @@ def test_batch_ragged_shapes_equal_alone(gpu):
         g.upload(h)
         g.run(cfdart.ALTERNATING, 15, cfl=0.8 if d.model == 0 else 0.5)
         assert g.checksum() == b.checksum(s), (d.model, d.nx, d.ny)
+
+
+@pytest.mark.parametrize("model", [0, 1])
+@pytest.mark.parametrize("yt", [cfdart.Y_TRANSPOSED, cfdart.Y_TRANSPOSED_NARROW, 5, 33])
+@pytest.mark.parametrize("xr", [1, 3, 7, 16])
+def test_tile_shapes(gpu, model, yt, xr):
+    """Every tile shape the autotuner may pick gives the oracle's bits (a grid that no tile
+    divides evenly)."""
+    d, h = state(model, 150, 97, cfdart.MUSCL_HANCOCK, seed=31 + model)
+    cfl = 0.8 if model == 0 else 0.5
+    print(f"case tile y={yt} x={xr} model={model}", flush=True)
+    g = cfdart.Grid(d)
+    g.set_tile(yt, xr)
+    g.upload(h)
+    t, n = g.run(cfdart.ALTERNATING, 6, cfl=cfl)
+    ora = oracle.Sim(d, h)
+    k, _ = ora.run(cfdart.ALTERNATING, 6, cfl=cfl)
+    assert n == k == 6 and t == ora.t
+    assert bitwise_equal(g.download()[2:-2, 2:-2], ora.grid[2:-2, 2:-2])
+    assert g.stats() == ora.counters()
