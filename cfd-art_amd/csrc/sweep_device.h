@@ -93,37 +93,30 @@ __device__ __forceinline__ int wave_sum(int v)
 struct Tally {
     double su = 0.0, sv = 0.0;  // running max of |u|+c, |v|+c (NaN never enters: smax keeps the old value)
     int slope = 0, evolve = 0, repair = 0, bad = 0;
-    // Workgroup-wide reduction, then one atomic per workgroup into a slot of the control block.
-    // Every thread of the workgroup must call it (it synchronises the workgroup).
+    // Wavefront-wide reduction, then lane 0 issues the atomics into a slot of the control block
+    // (slot = the workgroup's number times its wavefront count plus the wavefront's). Every lane
+    // of the wavefront must call it. There is no workgroup barrier: a barrier's release fence
+    // would hold the whole workgroup, and its LDS and wavefront slots, until every global store
+    // of its tile had completed (1.6 us per transposed y tile at 2048^2).
     __device__ void flush(Ctrl* c, bool last, int slot)
     {
-        __shared__ double s_m[2][16];
-        __shared__ int s_c[4][16];
-        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
         double mu = 0.0, mv = 0.0;
         if (last) { mu = wave_max(su); mv = wave_max(sv); }
-        int a = wave_sum(slope), b = wave_sum(evolve), d = wave_sum(repair), e = wave_sum(bad);
-        if (lane == 0) {
-            s_m[0][w] = mu; s_m[1][w] = mv;
-            s_c[0][w] = a; s_c[1][w] = b; s_c[2][w] = d; s_c[3][w] = e;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            for (int i = 1; i < nw; ++i) {
-                s_m[0][0] = smax(s_m[0][0], s_m[0][i]);
-                s_m[1][0] = smax(s_m[1][0], s_m[1][i]);
-                for (int k = 0; k < 4; ++k) s_c[k][0] += s_c[k][i];
-            }
-            const int sl = slot & (MAX_SLOTS - 1);
+        const int a = wave_sum(slope), b = wave_sum(evolve), d = wave_sum(repair), e = wave_sum(bad);
+        if ((threadIdx.x & 63) == 0) {
+            const int sl = (slot * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6)) & (MAX_SLOTS - 1);
             if (last) {
-                atomicMax(&c->maxu[sl], (unsigned long long)__double_as_longlong(s_m[0][0]));
-                atomicMax(&c->maxv[sl], (unsigned long long)__double_as_longlong(s_m[1][0]));
+                atomicMax(&c->maxu[sl], (unsigned long long)__double_as_longlong(mu));
+                atomicMax(&c->maxv[sl], (unsigned long long)__double_as_longlong(mv));
             }
-            for (int k = 0; k < 4; ++k)
-                if (s_c[k][0]) atomicAdd(&c->cnt[k], (unsigned long long)s_c[k][0]);
-            if (s_c[3][0]) atomicOr(&c->error, 1);
+            if (a) atomicAdd(&c->cnt[0], (unsigned long long)a);
+            if (b) atomicAdd(&c->cnt[1], (unsigned long long)b);
+            if (d) atomicAdd(&c->cnt[2], (unsigned long long)d);
+            if (e) {
+                atomicAdd(&c->cnt[3], (unsigned long long)e);
+                atomicOr(&c->error, 1);
+            }
         }
-        __syncthreads();
     }
 };
 
@@ -157,21 +150,94 @@ __device__ __forceinline__ void update_cell(const GridView& g, double r, const d
 
 // ------------------------------------------------------------------ x sweep
 
-// One x-sweep tile: the calling wavefront covers columns [tx*60-2, tx*60+62) and rows
-// [row0, row0+nrows); it writes the middle 60 columns.
+// One line of an x-sweep tile: the calling wavefront's lanes hold columns [tx*60-2, tx*60+62)
+// of row y (U: their conserved state as loaded, before the wall's sign flip); it writes the
+// middle 60 columns.
 template <int M, bool LAST>
-__device__ __forceinline__ void x_tile(const GridView& g, double dt, int tx, int row0, int nrows, Tally& tl)
+__device__ __forceinline__ void x_line(const GridView& g, double s, double r, int tx, int y, double* U, Tally& tl)
 {
     constexpr int NC = Model<M>::NC;
     constexpr int MN = Model<M>::MN;
     const int lane = threadIdx.x & 63;
     const int x = tx * XTILE - 2 + lane;
     const bool store = lane >= 2 && lane < 62 && x < g.nx;
-    const double s = 0.5 * (dt / g.dx);
-    const double r = dt / g.dx;
-    const int xs = mirror(x, g.nx);
     const bool neg = (x < 0 && g.bc[0]) || (x >= g.nx && g.bc[1]);
     const bool face_counted = (lane >= 2 && lane < 62 && x <= g.nx) || (lane == 62 && x == g.nx);
+    if (neg) U[MN] = -U[MN];
+
+    Prim<M> P;
+    P.from_cons(g.mat, U);
+    double Wm[NC], Wp[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) { Wm[k] = lane_from_left_or_self(P.W[k]); Wp[k] = lane_from_right_or_self(P.W[k]); }
+    CellStage<M> cs;
+    cell_stage<M>(g.mat, g.opt.limiter, s, P, Wm, Wp, cs);
+
+    // left neighbour's data for this lane's left face (lane 0, which has no left neighbour
+    // and whose face is not used, takes its own cell stage: a physical state, so it sends
+    // the wavefront into no fallback branch)
+    Prim<M> Pa;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) Pa.W[k] = Wm[k];
+    Pa.c = lane_from_left(P.c);
+    double Ua[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) Ua[k] = lane_from_left(U[k]);
+    CellStage<M> a;
+    a.fR = lane_from_left_or_self_i(cs.fR ? 1 : 0) != 0;
+    side_from_left<M>(cs.R, a.R);
+
+    FaceOut<M> fo;
+    int fl = face_stage<M>(g.mat, g.opt, s, Pa, Ua, a, P, U, cs, fo);
+
+    double Fr[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) Fr[k] = lane_from_right(fo.F[k]);
+    double usr = lane_from_right(fo.ustar);
+    double zsr = M == ALLAIRE ? lane_from_right(fo.zstar) : 0.0;
+
+    // lanes 0 and 63 (halo lanes without a neighbour on one side) update from incomplete
+    // fluxes and are not stored: the repair and the physical-state test skip them
+    double Un[NC];
+    bool bad;
+    update_cell<M>(g, r, U, fo.F, Fr, fo.ustar, usr, fo.zstar, zsr, Un, bad, store);
+    bool rep = false;
+    if (M == EULER && store) rep = euler_repair(g.mat, U, Un);
+
+    if (face_counted) { tl.slope += fl & 1; tl.evolve += (fl >> 1) & 1; }
+    if (store) {
+        double* out = g.nxt + (long long)y * g.pitch + x;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) __builtin_nontemporal_store(Un[k], out + k * g.plane);
+        tl.repair += rep;
+        tl.bad += bad;
+        if (LAST) {
+            double su, sv;
+            wave_speeds(g.mat, Un, su, sv, (Model<M>*)nullptr);
+            tl.su = smax(tl.su, su);
+            tl.sv = smax(tl.sv, sv);
+        }
+    }
+}
+
+// The state of the lanes' columns of row y of tile tx.
+template <int M>
+__device__ __forceinline__ void x_load(const GridView& g, int tx, int y, double* U)
+{
+    const int xs = mirror(tx * XTILE - 2 + (int)(threadIdx.x & 63), g.nx);
+    const double* base = g.cur + (long long)y * g.pitch + xs;
+#pragma unroll
+    for (int k = 0; k < Model<M>::NC; ++k) U[k] = base[k * g.plane];
+}
+
+// One x-sweep tile: the calling wavefront covers columns [tx*60-2, tx*60+62) and rows
+// [row0, row0+nrows); it writes the middle 60 columns.
+template <int M, bool LAST>
+__device__ __forceinline__ void x_tile(const GridView& g, double dt, int tx, int row0, int nrows, Tally& tl)
+{
+    constexpr int NC = Model<M>::NC;
+    const double s = 0.5 * (dt / g.dx);
+    const double r = dt / g.dx;
     const int yend = min(row0 + nrows, g.ny);
 
     // rows are independent: for Euler the next row's state is loaded while this one is computed;
@@ -179,81 +245,17 @@ __device__ __forceinline__ void x_tile(const GridView& g, double dt, int tx, int
     // wavefronts (the prefetch registers would spill at its three wavefronts per SIMD)
     constexpr bool PREFETCH = M == EULER;
     double Unext[NC];
-    if (PREFETCH && row0 < yend) {
-        const double* base = g.cur + (long long)row0 * g.pitch + xs;
-#pragma unroll
-        for (int k = 0; k < NC; ++k) Unext[k] = base[k * g.plane];
-    }
+    if (PREFETCH && row0 < yend) x_load<M>(g, tx, row0, Unext);
     for (int y = row0; y < yend; ++y) {
         double U[NC];
         if constexpr (PREFETCH) {
 #pragma unroll
             for (int k = 0; k < NC; ++k) U[k] = Unext[k];
-            if (y + 1 < yend) {
-                const double* base = g.cur + (long long)(y + 1) * g.pitch + xs;
-#pragma unroll
-                for (int k = 0; k < NC; ++k) Unext[k] = base[k * g.plane];
-            }
+            if (y + 1 < yend) x_load<M>(g, tx, y + 1, Unext);
         } else {
-            const double* base = g.cur + (long long)y * g.pitch + xs;
-#pragma unroll
-            for (int k = 0; k < NC; ++k) U[k] = base[k * g.plane];
+            x_load<M>(g, tx, y, U);
         }
-        if (neg) U[MN] = -U[MN];
-
-        Prim<M> P;
-        P.from_cons(g.mat, U);
-        double Wm[NC], Wp[NC];
-#pragma unroll
-        for (int k = 0; k < NC; ++k) { Wm[k] = lane_from_left_or_self(P.W[k]); Wp[k] = lane_from_right_or_self(P.W[k]); }
-        CellStage<M> cs;
-        cell_stage<M>(g.mat, g.opt.limiter, s, P, Wm, Wp, cs);
-
-        // left neighbour's data for this lane's left face (lane 0, which has no left neighbour
-        // and whose face is not used, takes its own cell stage: a physical state, so it sends
-        // the wavefront into no fallback branch)
-        Prim<M> Pa;
-#pragma unroll
-        for (int k = 0; k < NC; ++k) Pa.W[k] = Wm[k];
-        Pa.c = lane_from_left(P.c);
-        double Ua[NC];
-#pragma unroll
-        for (int k = 0; k < NC; ++k) Ua[k] = lane_from_left(U[k]);
-        CellStage<M> a;
-        a.fR = lane_from_left_or_self_i(cs.fR ? 1 : 0) != 0;
-        side_from_left<M>(cs.R, a.R);
-
-        FaceOut<M> fo;
-        int fl = face_stage<M>(g.mat, g.opt, s, Pa, Ua, a, P, U, cs, fo);
-
-        double Fr[NC];
-#pragma unroll
-        for (int k = 0; k < NC; ++k) Fr[k] = lane_from_right(fo.F[k]);
-        double usr = lane_from_right(fo.ustar);
-        double zsr = M == ALLAIRE ? lane_from_right(fo.zstar) : 0.0;
-
-        // lanes 0 and 63 (halo lanes without a neighbour on one side) update from incomplete
-        // fluxes and are not stored: the repair and the physical-state test skip them
-        double Un[NC];
-        bool bad;
-        update_cell<M>(g, r, U, fo.F, Fr, fo.ustar, usr, fo.zstar, zsr, Un, bad, store);
-        bool rep = false;
-        if (M == EULER && store) rep = euler_repair(g.mat, U, Un);
-
-        if (face_counted) { tl.slope += fl & 1; tl.evolve += (fl >> 1) & 1; }
-        if (store) {
-            double* out = g.nxt + (long long)y * g.pitch + x;
-#pragma unroll
-            for (int k = 0; k < NC; ++k) __builtin_nontemporal_store(Un[k], out + k * g.plane);
-            tl.repair += rep;
-            tl.bad += bad;
-            if (LAST) {
-                double su, sv;
-                wave_speeds(g.mat, Un, su, sv, (Model<M>*)nullptr);
-                tl.su = smax(tl.su, su);
-                tl.sv = smax(tl.sv, sv);
-            }
-        }
+        x_line<M, LAST>(g, s, r, tx, y, U, tl);
     }
 }
 

@@ -11,7 +11,7 @@
 // Both fuse the ghost fill (mirrored indexing) of E/sourcecode/euler_ppsweep.cpp:7-118, the
 // face fluxes, the conservative update, the Euler repair / Allaire z-update and snapping, and,
 // in the last sweep of a step, the per-cell wave speeds of compute_dt
-// (E/sourcecode/euler_timestep.cpp:14-25) reduced to one atomic max per workgroup.
+// (E/sourcecode/euler_timestep.cpp:14-25) reduced to one atomic max per wavefront.
 #include "sweep_device.h"
 
 #include <hip/hip_runtime.h>
@@ -123,6 +123,7 @@ __global__ __launch_bounds__(256) void reduce_dt_kernel(GridView g, double* part
     tl.su = su;
     tl.sv = sv;
     tl.flush(g.ctrl, true, blockIdx.x);
+    __syncthreads();
     if (threadIdx.x == 0) {
         partials[2 * blockIdx.x] = ((sm[0][0] + sm[0][1]) + (sm[0][2] + sm[0][3]));
         partials[2 * blockIdx.x + 1] = ((sm[1][0] + sm[1][1]) + (sm[1][2] + sm[1][3]));
