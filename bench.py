@@ -6,6 +6,13 @@ CFL 0.8 (the first 5 steps at 0.2, as the reference's compute_dt does). A step i
 time step (an x and a y sweep + the CFL reduction) on the device-resident grid; the state is
 uploaded before the timed region, dt never leaves the device.
 
+Single-grid workloads are timed in both arithmetic modes of the library: the exact one (every
+division and square root correctly rounded, bitwise the reference's arithmetic) and the fast one
+(divisions and square roots within a few ulp). The line reports the fast mode, with its
+per-field relative L1 and Linf against the exact mode's state after the same steps (the accuracy
+bar is 1e-10) and the exact mode's own figures beside it; --arith exact reports the exact mode
+alone.
+
 Prints one JSON line: cell-updates/s, plus the HBM roofline of the dominant sweep kernel
 (algorithmic bytes = one FP64 read + write of the state per interior cell = 64 B) timed with HIP
 events on the library's own stream, and the CPU baseline (the oracle on the host cores, a
@@ -17,6 +24,8 @@ import math
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 for p in (os.path.join(ROOT, "cfd-art_amd"), os.path.join(ROOT, "oracle")):
@@ -235,6 +244,9 @@ def main():
                     help="oracle steps of euler_2048 on one thread, about 10 s (0: skip)")
     ap.add_argument("--strip-rows", type=int, default=0)
     ap.add_argument("--tile", default="", help="Y_STRIP_ROWS,X_ROWS_PER_WAVE: fixed tile shape, no autotuning")
+    ap.add_argument("--arith", default="fast", choices=["fast", "exact"],
+                    help="single-grid workloads: fast = time the exact and the fast arithmetic and report the fast "
+                         "one with its accuracy against the exact state; exact = the exact arithmetic only")
     ap.add_argument("--workload", default="euler_2048",
                     choices=["euler_2048", "allaire_1024", "euler_200", "euler_slab", "godunov_2048", "batched",
                              "served"])
@@ -312,47 +324,65 @@ def main():
         tuned = {"y_strip_rows": yh, "x_rows_per_wave": xr}
     else:
         tuned = cfdart.autotune.tune_one(single["model"], d.nx, log=log, case=(d, h))
-    g = cfdart.Grid(d)
-    g.set_tile(tuned["y_strip_rows"], tuned["x_rows_per_wave"])
-    if args.strip_rows:
-        g.set_strip_rows(args.strip_rows)
-    g.upload(h)
+    bytes_per_launch = 16.0 * single["nc"] * cells  # one FP64 read + write of the state per cell
+
+    def timed(arith):
+        """The workload's timed run in one arithmetic mode: warm-up steps, clocks brought up on a
+        scratch copy, then the timed steps with HIP events around every sweep."""
+        g = cfdart.Grid(d)
+        g.set_tile(tuned["y_strip_rows"], tuned["x_rows_per_wave"])
+        g.set_arith(arith)
+        if args.strip_rows:
+            g.set_strip_rows(args.strip_rows)
+        g.upload(h)
+        g.run(cfdart.ALTERNATING, args.warmup, cfl=cfl)
+        # bring the GPU to its working clocks on a scratch copy of the same state, so that the
+        # timed steps of the measured grid (steps warmup .. warmup+steps of the run) are not taken
+        # at idle clocks; the measured grid itself is not advanced by this
+        scratch = cfdart.Grid(d)
+        scratch.set_tile(tuned["y_strip_rows"], tuned["x_rows_per_wave"])
+        scratch.set_arith(arith)
+        scratch.upload(h)
+        steps = args.steps
+        if math.isfinite(T_end):
+            # the scratch run to T also gives the number of steps the timed run to T takes
+            t_s, n_s = scratch.run(cfdart.ALTERNATING, 20000, T=T_end, cfl=cfl)
+            if t_s != T_end:
+                raise SystemExit(f"bench.py: {args.workload} did not reach t = {T_end} in 20000 steps")
+            steps = n_s
+            log(f"{args.workload}: the run to t = {T_end} takes {n_s} steps, all timed")
+        else:
+            scratch.run(cfdart.ALTERNATING, 100, cfl=cfl)
+        scratch.close()
+        g.set_profiling(True)
+        g.sync()
+        t0 = time.perf_counter()
+        g.run(cfdart.ALTERNATING, steps, T=T_end, cfl=cfl, sync=False)
+        g.sync()
+        el = time.perf_counter() - t0
+        prof = g.profile()
+        g.set_profiling(False)
+        t_sim, n_sim = g.run(cfdart.ALTERNATING, 0, T=T_end, cfl=cfl)
+        per_kernel = {}
+        for axis, (ms, nl) in prof.items():
+            if nl:
+                per_kernel[axis] = {"avg_us": 1e3 * ms / nl, "launches": nl,
+                                    "gbs": bytes_per_launch / (ms / nl * 1e-3) / 1e9}
+        state = g.download()
+        r = dict(value=cells * steps / el, steps=steps, el=el, kernels=per_kernel, counters=g.stats(),
+                 t_sim=t_sim, n_sim=n_sim, state=state)
+        g.close()
+        return r
+
     if not math.isfinite(T_end):
         log(f"{args.workload}: warm-up {args.warmup} steps, timed {args.steps} steps")
-    g.run(cfdart.ALTERNATING, args.warmup, cfl=cfl)
-    # bring the GPU to its working clocks on a scratch copy of the same state, so that the timed
-    # steps of the measured grid (steps warmup .. warmup+steps of the run) are not taken at idle
-    # clocks; the measured grid itself is not advanced by this
-    scratch = cfdart.Grid(d)
-    scratch.set_tile(tuned["y_strip_rows"], tuned["x_rows_per_wave"])
-    scratch.upload(h)
-    if math.isfinite(T_end):
-        # the scratch run to T also gives the number of steps the timed run to T takes
-        t_s, n_s = scratch.run(cfdart.ALTERNATING, 20000, T=T_end, cfl=cfl)
-        if t_s != T_end:
-            raise SystemExit(f"bench.py: {args.workload} did not reach t = {T_end} in 20000 steps")
-        args.steps = n_s
-        log(f"{args.workload}: the run to t = {T_end} takes {n_s} steps, all timed")
-    else:
-        scratch.run(cfdart.ALTERNATING, 100, cfl=cfl)
-    scratch.close()
-    g.set_profiling(True)
-    g.sync()
-    t0 = time.perf_counter()
-    g.run(cfdart.ALTERNATING, args.steps, T=T_end, cfl=cfl, sync=False)
-    g.sync()
-    el = time.perf_counter() - t0
-    prof = g.profile()
-    g.set_profiling(False)
-    t_sim, n_sim = g.run(cfdart.ALTERNATING, 0, T=T_end, cfl=cfl)
-    counters = g.stats()
-
-    bytes_per_launch = 16.0 * single["nc"] * cells  # one FP64 read + write of the state per cell
-    per_kernel = {}
-    for axis, (ms, nl) in prof.items():
-        if nl:
-            per_kernel[axis] = {"avg_us": 1e3 * ms / nl, "launches": nl,
-                                "gbs": bytes_per_launch / (ms / nl * 1e-3) / 1e9}
+    modes = ["exact", "fast"] if args.arith == "fast" else ["exact"]
+    res = {m: timed(cfdart.ARITH_FAST if m == "fast" else cfdart.ARITH_EXACT) for m in modes}
+    for m in modes:
+        log(f"{args.workload}: {m} arithmetic {res[m]['value'] / 1e9:.2f} G cell-updates/s, "
+            f"x {res[m]['kernels']['x']['avg_us']:.1f} us, y {res[m]['kernels']['y']['avg_us']:.1f} us")
+    head = res[modes[-1]]
+    per_kernel = head["kernels"]
     dom = max(per_kernel, key=lambda a: per_kernel[a]["avg_us"] * per_kernel[a]["launches"])
     transposed = (dom == "y" and tuned["y_strip_rows"] < 0 and not args.strip_rows
                   and d.flux_solver == cfdart.MUSCL_HANCOCK)  # the Godunov scheme runs y strips
@@ -360,29 +390,52 @@ def main():
     achieved = per_kernel[dom]["gbs"]
     traffic = read_traffic(args.workload, dom)
 
-    value = cells * args.steps / el
     out = {
         "metric": single["metric"],
-        "value": value,
+        "value": head["value"],
         "unit": "cell-updates/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": head["steps"],
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * el / args.steps,
+        "ms_per_step": 1e3 * head["el"] / head["steps"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": single["data"],
         "config": dict({"workload": args.workload, "nx": d.nx, "ny": d.ny}, **single["config"],
-                       splitting="alternating", cfl=cfl, sweeps_per_step=2, t_end=t_sim, steps_total=n_sim),
-        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": bytes_per_launch},
+                       splitting="alternating", cfl=cfl, sweeps_per_step=2, t_end=head["t_sim"],
+                       steps_total=head["n_sim"], arith=modes[-1]),
+        "roofline": {"bound": "hbm", "kernel": kname + (" (fast arithmetic)" if modes[-1] == "fast" else ""),
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic, "algorithmic_bytes_per_launch": bytes_per_launch},
         "kernels": per_kernel,
         "tile": {"y_strip_rows": tuned["y_strip_rows"], "x_rows_per_wave": tuned["x_rows_per_wave"]},
-        "fallback_counters": counters,
+        "fallback_counters": head["counters"],
     }
+    if modes[-1] == "fast":
+        ex = res["exact"]
+        ex_dom = ex["kernels"][dom]["gbs"]
+        out["exact_arith"] = {"value": ex["value"], "ms_per_step": 1e3 * ex["el"] / ex["steps"],
+                              "kernels": ex["kernels"], "roofline_frac": ex_dom / HBM_PEAK_GBS,
+                              "note": "every division and square root correctly rounded: bitwise the "
+                                      "reference's arithmetic (the oracle), the library's default"}
+        a, b = head["state"], ex["state"]
+        ng = d.ng
+        errs = []
+        if head["n_sim"] == ex["n_sim"]:
+            for k in range(a.shape[-1]):
+                x, y = a[ng:-ng, ng:-ng, k], b[ng:-ng, ng:-ng, k]
+                errs.append((float(np.abs(x - y).sum() / np.abs(y).sum()), float(np.abs(x - y).max() / np.abs(y).max())))
+        out["accuracy"] = {
+            "against": "the exact mode's state after the same steps (bitwise the CPU reference's arithmetic)",
+            "steps": head["n_sim"], "bar": 1e-10,
+            "rel_l1_per_field": [e[0] for e in errs], "rel_linf_per_field": [e[1] for e in errs],
+            "rel_l1_max": max(e[0] for e in errs) if errs else None,
+            "rel_linf_max": max(e[1] for e in errs) if errs else None,
+            "t_end_exact": ex["t_sim"]}
+        log(f"{args.workload}: fast vs exact after {head['n_sim']} steps: rel L1 max "
+            f"{out['accuracy']['rel_l1_max']:.2e}, rel Linf max {out['accuracy']['rel_linf_max']:.2e}")
     if args.cpu_steps > 0 and single.get("cpu"):
         c = single["cpu"]
         out["cpu_baseline"] = cpu_baseline(single["cfg"], cfl, c["steps"], c["steps_1t"], T=single.get("T", math.inf),

@@ -7,7 +7,7 @@ back to a CPU implementation: if the library or a GPU is missing, calls raise.
 from ._lib import (  # noqa: F401
     CfdartError, Desc, DiscParams, BubbleParams, Grid, Batch, time_tile, lib, library_path,
     EULER, ALLAIRE, TRANSMISSIVE, REFLECTIVE, MINMOD, VANLEER, GARRICK, PVRS,
-    NAIVE, STRANG, ALTERNATING, MUSCL_HANCOCK, GODUNOV, Y_TRANSPOSED, Y_TRANSPOSED_NARROW, ncomp, make_desc, random_discs, shocked_bubbles, planar,
+    NAIVE, STRANG, ALTERNATING, MUSCL_HANCOCK, GODUNOV, Y_TRANSPOSED, Y_TRANSPOSED_NARROW, ARITH_EXACT, ARITH_FAST, ncomp, make_desc, random_discs, shocked_bubbles, planar,
     prim_to_cons, device_count, state_checksum,
 )
 from . import autotune, configs, serve  # noqa: F401

@@ -13,6 +13,7 @@ NAIVE, STRANG, ALTERNATING = 0, 1, 2
 MUSCL_HANCOCK, GODUNOV = 0, 1
 Y_TRANSPOSED = -1  # set_tile: the transposed y tile, 16 columns (CFDART_Y_TRANSPOSED)
 Y_TRANSPOSED_NARROW = -2  # 8 columns (CFDART_Y_TRANSPOSED_NARROW)
+ARITH_EXACT, ARITH_FAST = 0, 1  # set_arith (CFDART_ARITH_EXACT, CFDART_ARITH_FAST)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 library_path = os.path.join(os.path.dirname(_HERE), "libcfdart.so")
@@ -91,6 +92,7 @@ def lib():
         L.cfdart_ic_planar.argtypes = [P(Desc), dp, dp, C.c_double, C.c_double, C.c_double, dp]
         L.cfdart_prim_to_cons.argtypes = [P(Desc), dp, dp]
         L.cfdart_set_tile.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.cfdart_set_arith.argtypes = [C.c_void_p, C.c_int]
         L.cfdart_init_random_discs.argtypes = [C.c_void_p, P(DiscParams)]
         L.cfdart_init_shocked_bubbles.argtypes = [C.c_void_p, P(BubbleParams)]
         L.cfdart_checksum.argtypes = [C.c_void_p, P(C.c_ulonglong)]
@@ -298,6 +300,10 @@ class Grid:
 
     def set_tile(self, y_strip_rows, x_rows_per_wave):
         _check(lib().cfdart_set_tile(self.h, y_strip_rows, x_rows_per_wave))
+
+    def set_arith(self, mode):
+        """ARITH_EXACT (default: bitwise the reference's arithmetic) or ARITH_FAST."""
+        _check(lib().cfdart_set_arith(self.h, mode))
 
     def set_strip_rows(self, rows):
         _check(lib().cfdart_set_strip_rows(self.h, rows))

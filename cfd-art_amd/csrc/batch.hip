@@ -471,6 +471,7 @@ int admit(cfdart_batch* b, const cfdart_desc* d, const double* host, const cfdar
     g.opt.z_lo = d->z_lo;
     g.opt.z_hi = d->z_hi;
     g.opt.godunov = 0;
+    g.opt.fast = 0;
     g.dx = d->dx;
     g.dy = d->dy;
     g.dx_half_col = d->colsweep_uses_dx ? d->dx : d->dy;

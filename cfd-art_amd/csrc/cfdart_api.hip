@@ -26,6 +26,7 @@ struct cfdart_grid {
     int nred;
     hipStream_t st;
     int yH, xrows;
+    int arith;  // CFDART_ARITH_EXACT or CFDART_ARITH_FAST
     // optional per-kernel timing: an event before and after every sweep launch
     int profiling;
     std::vector<hipEvent_t> ev_pool;
@@ -93,6 +94,7 @@ cfd::GridView view(const cfdart_grid* g)
     v.opt.z_lo = g->d.z_lo;
     v.opt.z_hi = g->d.z_hi;
     v.opt.godunov = g->d.flux_solver == CFDART_GODUNOV ? 1 : 0;
+    v.opt.fast = g->arith == CFDART_ARITH_FAST ? 1 : 0;
     v.dx = g->d.dx;
     v.dy = g->d.dy;
     v.dx_half_col = g->d.colsweep_uses_dx ? g->d.dx : g->d.dy;
@@ -199,6 +201,7 @@ int cfdart_create(const cfdart_desc* desc, cfdart_grid** out)
     g->cur = 0;
     g->yH = CFDART_Y_TRANSPOSED;
     g->xrows = 4;
+    g->arith = CFDART_ARITH_EXACT;
     g->buf[0] = g->buf[1] = nullptr;
     g->ctrl = nullptr;
     g->partials = nullptr;
@@ -560,6 +563,13 @@ int cfdart_set_tile(cfdart_grid* g, int y_strip_rows, int x_rows_per_wave)
         return fail(CFDART_E_ARG, "tile shape out of range");
     g->yH = y_strip_rows;
     g->xrows = x_rows_per_wave;
+    return 0;
+}
+
+int cfdart_set_arith(cfdart_grid* g, int mode)
+{
+    if (!g || (mode != CFDART_ARITH_EXACT && mode != CFDART_ARITH_FAST)) return fail(CFDART_E_ARG, "bad handle or arithmetic mode");
+    g->arith = mode;
     return 0;
 }
 

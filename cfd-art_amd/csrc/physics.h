@@ -34,31 +34,40 @@ namespace cfd {
 
 enum { EULER = 0, ALLAIRE = 1 };
 
-// FP64 division and square root, correctly rounded. On the device these are the steps of the
-// compiler's IEEE sequences without their operand scaling: v_rcp_f64 refined by two Newton
-// steps, the product, one residual correction and v_div_fixup for a / b; v_rsq_f64 refined by
-// Goldschmidt steps for sqrt. The scaling only acts on operands the solver's states never reach
-// (a divide whose numerator is below 2^-969 in magnitude, a subnormal divisor or quotient, an
-// exponent gap of 768 or more; a square root of a number below 2^-767), so every quotient and
-// root equals the IEEE one; v_div_fixup and the class test keep zeros, infinities and NaNs exact.
-CFD_HD double ddiv(double a, double b)
+// FP64 division and square root. F = false (the exact arithmetic, the default): correctly
+// rounded. On the device these are the steps of the compiler's IEEE sequences without their
+// operand scaling: v_rcp_f64 refined by two Newton steps, the product, one residual correction
+// and v_div_fixup for a / b; v_rsq_f64 refined by Goldschmidt steps for sqrt. The scaling only
+// acts on operands the solver's states never reach (a divide whose numerator is below 2^-969 in
+// magnitude, a subnormal divisor or quotient, an exponent gap of 768 or more; a square root of a
+// number below 2^-767), so every quotient and root equals the IEEE one; v_div_fixup and the class
+// test keep zeros, infinities and NaNs exact.
+// F = true (the fast arithmetic, device only): a * r with r = v_rcp_f64 refined by one Newton
+// step, and one Goldschmidt step after v_rsq_f64 -- within a few ulp of the correctly rounded
+// results, at 1 instead of 4 instructions per numerator once a divisor's reciprocal is formed.
+// The host always computes exactly.
+template <bool F = false> CFD_HD double ddiv(double a, double b)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
     double r = __builtin_amdgcn_rcp(b);
     double e = __builtin_fma(-b, r, 1.0);
     r = __builtin_fma(r, e, r);
-    e = __builtin_fma(-b, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    double q = a * r;
-    const double t = __builtin_fma(-b, q, a);
-    q = __builtin_fma(t, r, q);
-    return __builtin_amdgcn_div_fixup(q, b, a);
+    if constexpr (F) {
+        return a * r;
+    } else {
+        e = __builtin_fma(-b, r, 1.0);
+        r = __builtin_fma(r, e, r);
+        double q = a * r;
+        const double t = __builtin_fma(-b, q, a);
+        q = __builtin_fma(t, r, q);
+        return __builtin_amdgcn_div_fixup(q, b, a);
+    }
 #else
     return a / b;
 #endif
 }
 
-CFD_HD double dsqrt(double x)
+template <bool F = false> CFD_HD double dsqrt(double x)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
     const double y = __builtin_amdgcn_rsq(x);
@@ -66,11 +75,13 @@ CFD_HD double dsqrt(double x)
     double h = y * 0.5;  // 1 / (2 sqrt(x))
     const double r = __builtin_fma(-h, g, 0.5);
     g = __builtin_fma(g, r, g);
-    h = __builtin_fma(h, r, h);
-    double d = __builtin_fma(-g, g, x);
-    g = __builtin_fma(d, h, g);
-    d = __builtin_fma(-g, g, x);
-    g = __builtin_fma(d, h, g);
+    if constexpr (!F) {
+        h = __builtin_fma(h, r, h);
+        double d = __builtin_fma(-g, g, x);
+        g = __builtin_fma(d, h, g);
+        d = __builtin_fma(-g, g, x);
+        g = __builtin_fma(d, h, g);
+    }
     // +0, -0 and +inf are their own roots (class mask: -0, +0, +inf)
     return __builtin_amdgcn_class(x, 0x260) ? x : g;
 #else
@@ -117,6 +128,7 @@ struct SchemeOpts {
     int wave_speeds;  // 0 Garrick-Owkes-Regele (reference), 1 PVRS (Allaire)
     double z_lo, z_hi;
     int godunov;      // 1: first-order Godunov faces and upwind z update instead of MUSCL-Hancock
+    int fast;         // 1: the sweeps' fast arithmetic (ddiv<true>, dsqrt<true>)
 };
 
 CFD_HD double smin(double a, double b) { return (b < a) ? b : a; }
@@ -126,11 +138,11 @@ CFD_HD double smax(double a, double b) { return (a < b) ? b : a; }
 
 // Mixture eps and k (A/headers/mixturemodel.hpp:20-22), recomputed per call there; here per z.
 struct Mix { double eps, k; };
-CFD_HD Mix mixture(const Material& m, double z)
+template <bool F = false> CFD_HD Mix mixture(const Material& m, double z)
 {
     Mix r;
-    r.eps = ddiv(z, m.g1m1) + ddiv(1.0 - z, m.g2m1);
-    r.k = ddiv(ddiv(z * m.g1 * m.pinf1, m.g1m1) + ddiv((1.0 - z) * m.g2 * m.pinf2, m.g2m1), r.eps);
+    r.eps = ddiv<F>(z, m.g1m1) + ddiv<F>(1.0 - z, m.g2m1);
+    r.k = ddiv<F>(ddiv<F>(z * m.g1 * m.pinf1, m.g1m1) + ddiv<F>((1.0 - z) * m.g2 * m.pinf2, m.g2m1), r.eps);
     return r;
 }
 
@@ -138,18 +150,18 @@ template <int M> struct Eos;
 template <> struct Eos<EULER> {
     // E/headers/stiffened_gas_eos.hpp:57-70
     CFD_HD static double pressure(const Material& m, double rho, double e) { return m.g1m1 * rho * e - m.g1pinf1; }
-    CFD_HD static double specific_ie(const Material& m, double rho, double p) { return ddiv(p + m.g1pinf1, m.g1m1 * rho); }
-    CFD_HD static double soundspeed(const Material& m, double rho, double p) { return dsqrt(ddiv(m.g1 * (p + m.pinf1), rho)); }
+    template <bool F = false> CFD_HD static double specific_ie(const Material& m, double rho, double p) { return ddiv<F>(p + m.g1pinf1, m.g1m1 * rho); }
+    template <bool F = false> CFD_HD static double soundspeed(const Material& m, double rho, double p) { return dsqrt<F>(ddiv<F>(m.g1 * (p + m.pinf1), rho)); }
 };
 struct MixEos {
     // A/headers/mixturemodel.hpp:18-52
-    CFD_HD static double pressure(const Mix& x, double rho, double e) { return ddiv(rho * e, x.eps) - x.k; }
-    CFD_HD static double specific_ie(const Mix& x, double rho, double p) { return ddiv((p + x.k) * x.eps, rho); }
-    CFD_HD static double soundspeed(const Mix& x, double rho, double p) { return dsqrt(ddiv(x.k + p * (ddiv(1.0, x.eps) + 1.0), rho)); }
-    CFD_HD static double shock_jump(const Mix& x, double p, double rho, double pstar)
+    template <bool F = false> CFD_HD static double pressure(const Mix& x, double rho, double e) { return ddiv<F>(rho * e, x.eps) - x.k; }
+    template <bool F = false> CFD_HD static double specific_ie(const Mix& x, double rho, double p) { return ddiv<F>((p + x.k) * x.eps, rho); }
+    template <bool F = false> CFD_HD static double soundspeed(const Mix& x, double rho, double p) { return dsqrt<F>(ddiv<F>(x.k + p * (ddiv<F>(1.0, x.eps) + 1.0), rho)); }
+    template <bool F = false> CFD_HD static double shock_jump(const Mix& x, double p, double rho, double pstar)
     {
         double p_avg = 0.5 * (pstar + p);
-        return dsqrt(ddiv(pstar + x.k + ddiv(p_avg, x.eps), rho));
+        return dsqrt<F>(ddiv<F>(pstar + x.k + ddiv<F>(p_avg, x.eps), rho));
     }
 };
 
@@ -162,48 +174,48 @@ template <int M> struct Prim;
 template <> struct Prim<EULER> {
     double W[4];
     double c;
-    CFD_HD void from_cons(const Material& m, const double* U)
+    template <bool F = false> CFD_HD void from_cons(const Material& m, const double* U)
     {
         double rho = U[0];
-        double u = ddiv(U[1], rho);
-        double v = ddiv(U[2], rho);
-        double e = ddiv(U[3], rho) - 0.5 * (u * u + v * v);
+        double u = ddiv<F>(U[1], rho);
+        double v = ddiv<F>(U[2], rho);
+        double e = ddiv<F>(U[3], rho) - 0.5 * (u * u + v * v);
         double p = Eos<EULER>::pressure(m, rho, e);
         W[0] = rho; W[1] = u; W[2] = v; W[3] = p;
-        c = Eos<EULER>::soundspeed(m, rho, p);
+        c = Eos<EULER>::soundspeed<F>(m, rho, p);
     }
 };
 template <> struct Prim<ALLAIRE> {
     double W[6];
     double c;
-    CFD_HD void from_cons(const Material& m, const double* U)
+    template <bool F = false> CFD_HD void from_cons(const Material& m, const double* U)
     {
         double rho = U[0] + U[1];
-        double u = ddiv(U[2], rho);
-        double v = ddiv(U[3], rho);
-        double e = ddiv(U[4], rho) - 0.5 * (u * u + v * v);
-        Mix x = mixture(m, U[5]);
-        double p = MixEos::pressure(x, rho, e);
+        double u = ddiv<F>(U[2], rho);
+        double v = ddiv<F>(U[3], rho);
+        double e = ddiv<F>(U[4], rho) - 0.5 * (u * u + v * v);
+        Mix x = mixture<F>(m, U[5]);
+        double p = MixEos::pressure<F>(x, rho, e);
         W[0] = U[0]; W[1] = U[1]; W[2] = u; W[3] = v; W[4] = p; W[5] = U[5];
-        c = MixEos::soundspeed(x, rho, p);
+        c = MixEos::soundspeed<F>(x, rho, p);
     }
 };
 
 // Primitive -> conserved (E/headers/misc_euler.hpp:82-97, A/headers/misc.hpp:149-163).
-CFD_HD void p2c(const Material& m, const double* W, double* U, Model<EULER>*)
+template <bool F = false> CFD_HD void p2c(const Material& m, const double* W, double* U, Model<EULER>*)
 {
     double rho = W[0], u = W[1], v = W[2], p = W[3];
-    double e = Eos<EULER>::specific_ie(m, rho, p);
+    double e = Eos<EULER>::specific_ie<F>(m, rho, p);
     U[0] = rho;
     U[1] = rho * u;
     U[2] = rho * v;
     U[3] = rho * e + 0.5 * rho * (u * u + v * v);
 }
-CFD_HD void p2c(const Material& m, const double* W, double* U, Model<ALLAIRE>*)
+template <bool F = false> CFD_HD void p2c(const Material& m, const double* W, double* U, Model<ALLAIRE>*)
 {
     double rho = W[0] + W[1];
-    Mix x = mixture(m, W[5]);
-    double e = MixEos::specific_ie(x, rho, W[4]);
+    Mix x = mixture<F>(m, W[5]);
+    double e = MixEos::specific_ie<F>(x, rho, W[4]);
     U[0] = W[0];
     U[1] = W[1];
     U[2] = rho * W[2];
@@ -211,9 +223,9 @@ CFD_HD void p2c(const Material& m, const double* W, double* U, Model<ALLAIRE>*)
     U[4] = rho * e + 0.5 * rho * (W[2] * W[2] + W[3] * W[3]);
     U[5] = W[5];
 }
-template <int M> CFD_HD void p2c(const Material& m, const double* W, double* U)
+template <int M, bool F = false> CFD_HD void p2c(const Material& m, const double* W, double* U)
 {
-    p2c(m, W, U, (Model<M>*)nullptr);
+    p2c<F>(m, W, U, (Model<M>*)nullptr);
 }
 
 // Allaire snapping of the non-const is_physical_state overload (A/headers/misc.hpp:253-256).
@@ -227,24 +239,24 @@ CFD_HD void snap(double* U)
 
 // Physical-state test (E/headers/misc_euler.hpp:123-131, A/headers/misc.hpp:228-237). Euler
 // has no snapping; for Allaire the caller snaps first where the reference does.
-CFD_HD bool is_physical(const double* U, Model<EULER>*)
+template <bool F = false> CFD_HD bool is_physical(const double* U, Model<EULER>*)
 {
     double rho = U[0];
-    double u = ddiv(U[1], rho);
-    double v = ddiv(U[2], rho);
-    double e = ddiv(U[3], rho) - 0.5 * (u * u + v * v);
+    double u = ddiv<F>(U[1], rho);
+    double v = ddiv<F>(U[2], rho);
+    double e = ddiv<F>(U[3], rho) - 0.5 * (u * u + v * v);
     return rho >= 0.0 && e >= 0.0;
 }
-CFD_HD bool is_physical(const double* U, Model<ALLAIRE>*)
+template <bool F = false> CFD_HD bool is_physical(const double* U, Model<ALLAIRE>*)
 {
     double rho = U[0] + U[1];
-    double u = ddiv(U[2], rho);
-    double v = ddiv(U[3], rho);
-    double e = ddiv(U[4], rho) - 0.5 * (u * u + v * v);
+    double u = ddiv<F>(U[2], rho);
+    double v = ddiv<F>(U[3], rho);
+    double e = ddiv<F>(U[4], rho) - 0.5 * (u * u + v * v);
     double z = U[5];
     return U[0] >= 0.0 && U[1] >= 0.0 && e >= 0.0 && z >= 0.0 && z <= 1.0;
 }
-template <int M> CFD_HD bool is_physical(const double* U) { return is_physical(U, (Model<M>*)nullptr); }
+template <int M, bool F = false> CFD_HD bool is_physical(const double* U) { return is_physical<F>(U, (Model<M>*)nullptr); }
 
 // Division-free screens for the physical-state tests. Each returns true only when the exact
 // test (with its divisions and roundings) is certain to return true; otherwise the caller runs
@@ -280,19 +292,19 @@ CFD_HD bool cons_clearly_physical(double rho, double mu, double mv, double E)
 }
 
 // Allaire post-update test (after snapping): a division-free screen, then the exact test.
-CFD_HD bool allaire_physical_screened(const double* U)
+template <bool F = false> CFD_HD bool allaire_physical_screened(const double* U)
 {
     if (U[0] >= 0.0 && U[1] >= 0.0 && U[5] >= 0.0 && U[5] <= 1.0 &&
         cons_clearly_physical(U[0] + U[1], U[2], U[3], U[4]))
         return true;
-    return is_physical<ALLAIRE>(U);
+    return is_physical<ALLAIRE, F>(U);
 }
 
 // ---------------------------------------------------------------- limiter
 
-CFD_HD double limit(int limiter, double a, double b)
+template <bool F = false> CFD_HD double limit(int limiter, double a, double b)
 {
-    if (limiter == 1) return (a * b > 0.0) ? ddiv(2.0 * a * b, a + b) : 0.0;
+    if (limiter == 1) return (a * b > 0.0) ? ddiv<F>(2.0 * a * b, a + b) : 0.0;
     if (b > 0.0) return smax(0.0, smin(a, b));
     return smin(0.0, smax(a, b));
 }
@@ -312,64 +324,64 @@ template <int M> struct Side {
 
 // From a conserved state: HLLC's own primitive recovery (identical to the physical-state
 // test's divisions, which the compiler shares).
-CFD_HD void side_from_cons(const Material& m, Side<EULER>& s)
+template <bool F = false> CFD_HD void side_from_cons(const Material& m, Side<EULER>& s)
 {
     const double* U = s.U;
     double rho = U[0];
-    double u = ddiv(U[1], rho);
-    double v = ddiv(U[2], rho);
-    double e = ddiv(U[3], rho) - 0.5 * (u * u + v * v);
+    double u = ddiv<F>(U[1], rho);
+    double v = ddiv<F>(U[2], rho);
+    double e = ddiv<F>(U[3], rho) - 0.5 * (u * u + v * v);
     double p = Eos<EULER>::pressure(m, rho, e);
     s.rho = rho; s.u = u; s.p = p;
-    s.c = Eos<EULER>::soundspeed(m, rho, p);
+    s.c = Eos<EULER>::soundspeed<F>(m, rho, p);
     s.ok = rho >= 0.0 && e >= 0.0;
 }
-CFD_HD void side_from_cons(const Material& m, Side<ALLAIRE>& s)
+template <bool F = false> CFD_HD void side_from_cons(const Material& m, Side<ALLAIRE>& s)
 {
     const double* U = s.U;
     double rho = U[0] + U[1];
-    double u = ddiv(U[2], rho);
-    double v = ddiv(U[3], rho);
-    double e = ddiv(U[4], rho) - 0.5 * (u * u + v * v);
+    double u = ddiv<F>(U[2], rho);
+    double v = ddiv<F>(U[3], rho);
+    double e = ddiv<F>(U[4], rho) - 0.5 * (u * u + v * v);
     double z = U[5];
-    Mix x = mixture(m, z);
-    double p = MixEos::pressure(x, rho, e);
+    Mix x = mixture<F>(m, z);
+    double p = MixEos::pressure<F>(x, rho, e);
     s.rho = rho; s.u = u; s.p = p;
-    s.c = MixEos::soundspeed(x, rho, p);
+    s.c = MixEos::soundspeed<F>(x, rho, p);
     s.eps = x.eps; s.k = x.k;
     s.ok = U[0] >= 0.0 && U[1] >= 0.0 && e >= 0.0 && z >= 0.0 && z <= 1.0;
 }
 
 // HLLC inputs of a conserved Allaire state whose mixture coefficients are already known.
-CFD_HD void side_from_cons_mix(const Mix& x, Side<ALLAIRE>& s)
+template <bool F = false> CFD_HD void side_from_cons_mix(const Mix& x, Side<ALLAIRE>& s)
 {
     const double* U = s.U;
     double rho = U[0] + U[1];
-    double u = ddiv(U[2], rho);
-    double v = ddiv(U[3], rho);
-    double e = ddiv(U[4], rho) - 0.5 * (u * u + v * v);
+    double u = ddiv<F>(U[2], rho);
+    double v = ddiv<F>(U[3], rho);
+    double e = ddiv<F>(U[4], rho) - 0.5 * (u * u + v * v);
     double z = U[5];
-    double p = MixEos::pressure(x, rho, e);
+    double p = MixEos::pressure<F>(x, rho, e);
     s.rho = rho; s.u = u; s.p = p;
-    s.c = MixEos::soundspeed(x, rho, p);
+    s.c = MixEos::soundspeed<F>(x, rho, p);
     s.eps = x.eps; s.k = x.k;
     s.ok = U[0] >= 0.0 && U[1] >= 0.0 && e >= 0.0 && z >= 0.0 && z <= 1.0;
 }
 
 // Evolved state from its primitive vector T (step M4 result): p2c, then the M5 test; the
 // Allaire test snaps the state in place and the snapped state is what HLLC receives.
-CFD_HD void side_from_prim(const Material& m, const double* T, Side<EULER>& s)
+template <bool F = false> CFD_HD void side_from_prim_(const Material& m, const double* T, Side<EULER>& s)
 {
-    p2c<EULER>(m, T, s.U);
-    side_from_cons(m, s);
+    p2c<EULER, F>(m, T, s.U);
+    side_from_cons<F>(m, s);
 }
-CFD_HD void side_from_prim(const Material& m, const double* T, Side<ALLAIRE>& s)
+template <bool F = false> CFD_HD void side_from_prim_(const Material& m, const double* T, Side<ALLAIRE>& s)
 {
     // p2c with the mixture of z = T[5]; the same coefficients serve HLLC unless snapping
     // changes z (then they are recomputed from the snapped z, as the reference does)
     double rho = T[0] + T[1];
-    Mix x = mixture(m, T[5]);
-    double e = MixEos::specific_ie(x, rho, T[4]);
+    Mix x = mixture<F>(m, T[5]);
+    double e = MixEos::specific_ie<F>(x, rho, T[4]);
     s.U[0] = T[0];
     s.U[1] = T[1];
     s.U[2] = rho * T[2];
@@ -377,19 +389,19 @@ CFD_HD void side_from_prim(const Material& m, const double* T, Side<ALLAIRE>& s)
     s.U[4] = rho * e + 0.5 * rho * (T[2] * T[2] + T[3] * T[3]);
     s.U[5] = T[5];
     snap(s.U);
-    if (s.U[5] != T[5]) x = mixture(m, s.U[5]);
-    side_from_cons_mix(x, s);
+    if (s.U[5] != T[5]) x = mixture<F>(m, s.U[5]);
+    side_from_cons_mix<F>(x, s);
 }
-template <int M> CFD_HD void side_from_prim(const Material& m, const double* T, Side<M>& s)
+template <int M, bool F = false> CFD_HD void side_from_prim(const Material& m, const double* T, Side<M>& s)
 {
-    side_from_prim(m, T, s);
+    side_from_prim_<F>(m, T, s);
 }
 
 // The first-order fallback state (M5 failed): the raw stencil cell, unsnapped.
-template <int M> CFD_HD void side_from_raw(const Material& m, const double* U, Side<M>& s)
+template <int M, bool F = false> CFD_HD void side_from_raw(const Material& m, const double* U, Side<M>& s)
 {
     for (int k = 0; k < Model<M>::NC; ++k) s.U[k] = U[k];
-    side_from_cons(m, s);
+    side_from_cons<F>(m, s);
 }
 
 // ---------------------------------------------------------------- cell stage
@@ -404,23 +416,23 @@ template <int M> struct CellStage {
 
 // tmp = (s * A(W)) * del, summed over k = 0.. in order (Eigen's unrolled lazy product of a
 // fixed-size matrix and a vector: E/headers/flux_solver_MUSCLhancock.hpp:197,201).
-CFD_HD void jac_times(const double* W, double c, double s, const double* d, double* tmp, Model<EULER>*)
+template <bool F = false> CFD_HD void jac_times(const double* W, double c, double s, const double* d, double* tmp, Model<EULER>*)
 {
     double rho = W[0], u = W[1];
     double a00 = s * u, a01 = s * rho, z0 = s * 0.0;
-    double a13 = s * ddiv(1.0, rho);
+    double a13 = s * ddiv<F>(1.0, rho);
     double a31 = s * (rho * c * c);
     tmp[0] = z0 * d[3] + (z0 * d[2] + (a01 * d[1] + a00 * d[0]));
     tmp[1] = a13 * d[3] + (z0 * d[2] + (a00 * d[1] + z0 * d[0]));
     tmp[2] = z0 * d[3] + (a00 * d[2] + (z0 * d[1] + z0 * d[0]));
     tmp[3] = a00 * d[3] + (z0 * d[2] + (a31 * d[1] + z0 * d[0]));
 }
-CFD_HD void jac_times(const double* W, double c, double s, const double* d, double* tmp, Model<ALLAIRE>*)
+template <bool F = false> CFD_HD void jac_times(const double* W, double c, double s, const double* d, double* tmp, Model<ALLAIRE>*)
 {
     double rho = W[0] + W[1];
     double su = s * W[2], z0 = s * 0.0;
     double a02 = s * W[0], a12 = s * W[1];
-    double a24 = s * ddiv(1.0, rho);
+    double a24 = s * ddiv<F>(1.0, rho);
     double a42 = s * (rho * c * c);
     tmp[0] = z0 * d[5] + (z0 * d[4] + (z0 * d[3] + (a02 * d[2] + (z0 * d[1] + su * d[0]))));
     tmp[1] = z0 * d[5] + (z0 * d[4] + (z0 * d[3] + (a12 * d[2] + (su * d[1] + z0 * d[0]))));
@@ -431,57 +443,57 @@ CFD_HD void jac_times(const double* W, double c, double s, const double* d, doub
 }
 
 // M3 test of one extrapolated state (Allaire: the snapping overload on a temporary).
-CFD_HD bool extrap_physical(const Material& m, const double* T, Model<EULER>*)
+template <bool F = false> CFD_HD bool extrap_physical(const Material& m, const double* T, Model<EULER>*)
 {
     if (prim_clearly_physical(T[0], T[1], T[2], T[3], m.g1pinf1, m.g1m1)) return true;
     double U[4];
-    p2c<EULER>(m, T, U);
-    return is_physical<EULER>(U);
+    p2c<EULER, F>(m, T, U);
+    return is_physical<EULER, F>(U);
 }
-CFD_HD bool extrap_physical(const Material& m, const double* T, Model<ALLAIRE>*)
+template <bool F = false> CFD_HD bool extrap_physical(const Material& m, const double* T, Model<ALLAIRE>*)
 {
     // no snapping can occur when both partial densities and z are away from 0 (and z from 1)
     if (T[0] >= 1e-12 && T[1] >= 1e-12 && T[5] >= 1e-12 && T[5] <= 1.0 - 1e-12 &&
-        prim_clearly_physical(T[0] + T[1], T[2], T[3], T[4], m.kmin, ddiv(1.0, m.epsmin)))
+        prim_clearly_physical(T[0] + T[1], T[2], T[3], T[4], m.kmin, ddiv<F>(1.0, m.epsmin)))
         return true;
     double U[6];
-    p2c<ALLAIRE>(m, T, U);
+    p2c<ALLAIRE, F>(m, T, U);
     snap(U);
-    return is_physical<ALLAIRE>(U);
+    return is_physical<ALLAIRE, F>(U);
 }
-template <int M> CFD_HD bool extrap_physical(const Material& m, const double* T)
+template <int M, bool F = false> CFD_HD bool extrap_physical(const Material& m, const double* T)
 {
-    return extrap_physical(m, T, (Model<M>*)nullptr);
+    return extrap_physical<F>(m, T, (Model<M>*)nullptr);
 }
 
 // Cell stage from the cell's primitives W (with c) and its two neighbours' primitives.
-template <int M>
+template <int M, bool F = false>
 CFD_HD void cell_stage(const Material& m, int limiter, double s, const Prim<M>& P,
                        const double* Wm, const double* Wp, CellStage<M>& cs)
 {
     constexpr int NC = Model<M>::NC;
     const double* W = P.W;
     double del[NC], TR[NC], TL[NC], tmp[NC];
-    for (int k = 0; k < NC; ++k) del[k] = limit(limiter, W[k] - Wm[k], Wp[k] - W[k]);
+    for (int k = 0; k < NC; ++k) del[k] = limit<F>(limiter, W[k] - Wm[k], Wp[k] - W[k]);
     for (int k = 0; k < NC; ++k) { TR[k] = W[k] + 0.5 * del[k]; TL[k] = W[k] - 0.5 * del[k]; }
-    cs.fR = extrap_physical<M>(m, TR);
-    cs.fL = extrap_physical<M>(m, TL);
-    jac_times(W, P.c, s, del, tmp, (Model<M>*)nullptr);
+    cs.fR = extrap_physical<M, F>(m, TR);
+    cs.fL = extrap_physical<M, F>(m, TL);
+    jac_times<F>(W, P.c, s, del, tmp, (Model<M>*)nullptr);
     for (int k = 0; k < NC; ++k) { TR[k] = TR[k] - tmp[k]; TL[k] = TL[k] - tmp[k]; }
-    side_from_prim<M>(m, TR, cs.R);
-    side_from_prim<M>(m, TL, cs.L);
+    side_from_prim<M, F>(m, TR, cs.R);
+    side_from_prim<M, F>(m, TL, cs.L);
 }
 
 // Unsloped half-step state (slopes zeroed by M3): W + 0.5*0 - (s*A)*0, evaluated literally.
-template <int M>
+template <int M, bool F = false>
 CFD_HD void unsloped_side(const Material& m, double s, const Prim<M>& P, bool right, Side<M>& out)
 {
     constexpr int NC = Model<M>::NC;
     double del[NC], T[NC], tmp[NC];
     for (int k = 0; k < NC; ++k) del[k] = 0.0;
-    jac_times(P.W, P.c, s, del, tmp, (Model<M>*)nullptr);
+    jac_times<F>(P.W, P.c, s, del, tmp, (Model<M>*)nullptr);
     for (int k = 0; k < NC; ++k) T[k] = (right ? P.W[k] + 0.5 * del[k] : P.W[k] - 0.5 * del[k]) - tmp[k];
-    side_from_prim<M>(m, T, out);
+    side_from_prim<M, F>(m, T, out);
 }
 
 // ---------------------------------------------------------------- HLLC
@@ -493,7 +505,7 @@ template <int M> struct FaceOut {
 
 // HLLC flux from two prepared sides (E/headers/riemann_solver_HLLC.hpp:69-144,
 // A/headers/riemann_solver_HLLC.hpp:70-149) and z* (A/headers/flux_solver_MUSCLhancock.hpp:375).
-template <int M>
+template <int M, bool F = false>
 CFD_HD void hllc(const Material& m, int wave_speeds, const Side<M>& L, const Side<M>& R, FaceOut<M>& o)
 {
     constexpr int NC = Model<M>::NC;
@@ -510,9 +522,9 @@ CFD_HD void hllc(const Material& m, int wave_speeds, const Side<M>& L, const Sid
         Mix xL, xR;
         xL.eps = L.eps; xL.k = L.k;
         xR.eps = R.eps; xR.k = R.k;
-        if (p_pvrs > pL) SL = uL - MixEos::shock_jump(xL, pL, rhoL, p_pvrs);
+        if (p_pvrs > pL) SL = uL - MixEos::shock_jump<F>(xL, pL, rhoL, p_pvrs);
         else SL = uL - cL;
-        if (p_pvrs > pR) SR = uR + MixEos::shock_jump(xR, pR, rhoR, p_pvrs);
+        if (p_pvrs > pR) SR = uR + MixEos::shock_jump<F>(xR, pR, rhoR, p_pvrs);
         else SR = uR + cR;
     } else {
         double u_avg = 0.5 * (uL + uR);
@@ -520,17 +532,17 @@ CFD_HD void hllc(const Material& m, int wave_speeds, const Side<M>& L, const Sid
         SL = smin(u_avg - c_avg, uL - cL);
         SR = smax(u_avg + c_avg, uR + cR);
     }
-    double us = ddiv(pR - pL + rhoL * uL * (SL - uL) - rhoR * uR * (SR - uR),
+    double us = ddiv<F>(pR - pL + rhoL * uL * (SL - uL) - rhoR * uR * (SR - uR),
                      rhoL * (SL - uL) - rhoR * (SR - uR));
-    double fL = ddiv(SL - uL, SL - us);
-    double fR = ddiv(SR - uR, SR - us);
+    double fL = ddiv<F>(SL - uL, SL - us);
+    double fR = ddiv<F>(SR - uR, SR - us);
     double sg = copysign(1.0, us);
     double wL = (1.0 + sg) / 2.0;
     double wR = (1.0 - sg) / 2.0;
     double mSL = smin(0.0, SL);
     double MSR = smax(0.0, SR);
-    double eL = L.U[NC == 4 ? 3 : 4] + (us - uL) * (rhoL * us + ddiv(pL, SL - uL));
-    double eR = R.U[NC == 4 ? 3 : 4] + (us - uR) * (rhoR * us + ddiv(pR, SR - uR));
+    double eL = L.U[NC == 4 ? 3 : 4] + (us - uL) * (rhoL * us + ddiv<F>(pL, SL - uL));
+    double eR = R.U[NC == 4 ? 3 : 4] + (us - uR) * (rhoR * us + ddiv<F>(pR, SR - uR));
     for (int k = 0; k < NC; ++k) {
         double usl, usr, fl, fr;
         if (M == ALLAIRE && k == 5) {
@@ -564,7 +576,7 @@ CFD_HD void hllc(const Material& m, int wave_speeds, const Side<M>& L, const Sid
 // Face between cell a (left) and cell b (right): M3 combination, M5 fallback, HLLC.
 // flags: bit0 slope fallback taken, bit1 evolve fallback taken. The two fallbacks only choose
 // the Riemann-problem inputs; there is one HLLC evaluation per face.
-template <int M>
+template <int M, bool F = false>
 CFD_HD int face_stage(const Material& m, const SchemeOpts& o, double s,
                       const Prim<M>& Pa, const double* Ua, const CellStage<M>& a,
                       const Prim<M>& Pb, const double* Ub, const CellStage<M>& b, FaceOut<M>& out)
@@ -573,38 +585,38 @@ CFD_HD int face_stage(const Material& m, const SchemeOpts& o, double s,
     Side<M> L = a.R, R = b.L;
     if (!(a.fR && b.fL)) {
         flags |= 1;
-        unsloped_side<M>(m, s, Pa, true, L);
-        unsloped_side<M>(m, s, Pb, false, R);
+        unsloped_side<M, F>(m, s, Pa, true, L);
+        unsloped_side<M, F>(m, s, Pb, false, R);
     }
     if (!(L.ok && R.ok)) {
         flags |= 2;
-        side_from_raw<M>(m, Ua, L);
-        side_from_raw<M>(m, Ub, R);
+        side_from_raw<M, F>(m, Ua, L);
+        side_from_raw<M, F>(m, Ub, R);
     }
-    hllc<M>(m, o.wave_speeds, L, R, out);
+    hllc<M, F>(m, o.wave_speeds, L, R, out);
     return flags;
 }
 
 // ---------------------------------------------------------------- update
 
 // Euler repair (E/sourcecode/euler_rowsweep.cpp:47-82): 'old' and 'fut' in the unrotated frame.
-CFD_HD bool euler_repair(const Material& m, const double* old, double* fut)
+template <bool F = false> CFD_HD bool euler_repair(const Material& m, const double* old, double* fut)
 {
     if (cons_clearly_physical(fut[0], fut[1], fut[2], fut[3])) return false;
     double rho = fut[0];
-    double u = ddiv(fut[1], rho);
-    double v = ddiv(fut[2], rho);
-    double e = ddiv(fut[3], rho) - 0.5 * (u * u + v * v);
+    double u = ddiv<F>(fut[1], rho);
+    double v = ddiv<F>(fut[2], rho);
+    double e = ddiv<F>(fut[3], rho) - 0.5 * (u * u + v * v);
     if (rho >= 0.0 && e >= 0.0) return false;
     double rho_old = old[0];
-    double u_old = ddiv(old[1], rho_old);
-    double v_old = ddiv(old[2], rho_old);
-    double e_old = ddiv(old[3], rho_old) - 0.5 * (u_old * u_old + v_old * v_old);
+    double u_old = ddiv<F>(old[1], rho_old);
+    double v_old = ddiv<F>(old[2], rho_old);
+    double e_old = ddiv<F>(old[3], rho_old) - 0.5 * (u_old * u_old + v_old * v_old);
     if (rho <= 1e-12) rho = rho_old;
     if (e <= 1e-12) e = e_old;
     double p = Eos<EULER>::pressure(m, rho, e);
     double W[4] = {rho, u, v, p};
-    p2c<EULER>(m, W, fut);
+    p2c<EULER, F>(m, W, fut);
     return true;
 }
 
@@ -626,38 +638,38 @@ CFD_HD double zupdate_upwind(double r, double zL, double z, double zR, double us
 
 // Godunov face (E/headers/flux_solver_godunov.hpp:30-36, A/headers/flux_solver_godunov.hpp:30-39):
 // HLLC of the two cells' own conserved states; z* as in hllc.
-template <int M>
+template <int M, bool F = false>
 CFD_HD void godunov_face(const Material& m, int wave_speeds, const double* Ua, const double* Ub, FaceOut<M>& out)
 {
     Side<M> L, R;
-    side_from_raw<M>(m, Ua, L);
-    side_from_raw<M>(m, Ub, R);
-    hllc<M>(m, wave_speeds, L, R, out);
+    side_from_raw<M, F>(m, Ua, L);
+    side_from_raw<M, F>(m, Ub, R);
+    hllc<M, F>(m, wave_speeds, L, R, out);
 }
 
 // Wave speed |u|+c, |v|+c of a final (unrotated) state: compute_dt's per-cell work
 // (E/sourcecode/euler_timestep.cpp:16-24, A/sourcecode/allaire_diffuse_timestep.cpp:17-26).
-CFD_HD void wave_speeds(const Material& m, const double* U, double& su, double& sv, Model<EULER>*)
+template <bool F = false> CFD_HD void wave_speeds(const Material& m, const double* U, double& su, double& sv, Model<EULER>*)
 {
     double rho = U[0];
-    double u = ddiv(U[1], rho);
-    double v = ddiv(U[2], rho);
-    double e = ddiv(U[3], rho) - 0.5 * (u * u + v * v);
+    double u = ddiv<F>(U[1], rho);
+    double v = ddiv<F>(U[2], rho);
+    double e = ddiv<F>(U[3], rho) - 0.5 * (u * u + v * v);
     double p = Eos<EULER>::pressure(m, rho, e);
-    double c = Eos<EULER>::soundspeed(m, rho, p);
+    double c = Eos<EULER>::soundspeed<F>(m, rho, p);
     su = fabs(u) + c;
     sv = fabs(v) + c;
 }
-CFD_HD void wave_speeds(const Material& m, const double* U, double& su, double& sv, Model<ALLAIRE>*)
+template <bool F = false> CFD_HD void wave_speeds(const Material& m, const double* U, double& su, double& sv, Model<ALLAIRE>*)
 {
     double rho = U[0] + U[1];
-    double u = ddiv(U[2], rho);
-    double v = ddiv(U[3], rho);
-    double e = ddiv(U[4], rho) - 0.5 * (u * u + v * v);
+    double u = ddiv<F>(U[2], rho);
+    double v = ddiv<F>(U[3], rho);
+    double e = ddiv<F>(U[4], rho) - 0.5 * (u * u + v * v);
     double z = U[5];
-    Mix x = mixture(m, z);
-    double p = MixEos::pressure(x, rho, e);
-    double c = MixEos::soundspeed(x, rho, p);
+    Mix x = mixture<F>(m, z);
+    double p = MixEos::pressure<F>(x, rho, e);
+    double c = MixEos::soundspeed<F>(x, rho, p);
     su = fabs(u) + c;
     sv = fabs(v) + c;
 }

@@ -132,7 +132,7 @@ __device__ __forceinline__ bool sweep_dt(const GridView& g, double dt_arg, doubl
 // Conservative update + model-specific post-processing of one cell (row-sweep frame or
 // rotated column frame): E/sourcecode/euler_rowsweep.cpp:36-39,
 // A/sourcecode/allaire_diffuse_rowsweep.cpp:36-50.
-template <int M>
+template <int M, bool F = false>
 __device__ __forceinline__ void update_cell(const GridView& g, double r, const double* U,
                                            const double* FL, const double* FR, double usL, double usR,
                                            double zsL, double zsR, double* Un, bool& bad, bool checked = true)
@@ -144,7 +144,7 @@ __device__ __forceinline__ void update_cell(const GridView& g, double r, const d
     if (M == ALLAIRE) {
         Un[5] = zupdate(r, U[5], usL, usR, zsL, zsR, g.opt.z_lo, g.opt.z_hi);
         snap(Un);
-        bad = checked && !allaire_physical_screened(Un);
+        bad = checked && !allaire_physical_screened<F>(Un);
     }
 }
 
@@ -153,7 +153,7 @@ __device__ __forceinline__ void update_cell(const GridView& g, double r, const d
 // One line of an x-sweep tile: the calling wavefront's lanes hold columns [tx*60-2, tx*60+62)
 // of row y (U: their conserved state as loaded, before the wall's sign flip); it writes the
 // middle 60 columns.
-template <int M, bool LAST>
+template <int M, bool LAST, bool F = false>
 __device__ __forceinline__ void x_line(const GridView& g, double s, double r, int tx, int y, double* U, Tally& tl)
 {
     constexpr int NC = Model<M>::NC;
@@ -166,12 +166,12 @@ __device__ __forceinline__ void x_line(const GridView& g, double s, double r, in
     if (neg) U[MN] = -U[MN];
 
     Prim<M> P;
-    P.from_cons(g.mat, U);
+    P.template from_cons<F>(g.mat, U);
     double Wm[NC], Wp[NC];
 #pragma unroll
     for (int k = 0; k < NC; ++k) { Wm[k] = lane_from_left_or_self(P.W[k]); Wp[k] = lane_from_right_or_self(P.W[k]); }
     CellStage<M> cs;
-    cell_stage<M>(g.mat, g.opt.limiter, s, P, Wm, Wp, cs);
+    cell_stage<M, F>(g.mat, g.opt.limiter, s, P, Wm, Wp, cs);
 
     // left neighbour's data for this lane's left face (lane 0, which has no left neighbour
     // and whose face is not used, takes its own cell stage: a physical state, so it sends
@@ -188,7 +188,7 @@ __device__ __forceinline__ void x_line(const GridView& g, double s, double r, in
     side_from_left<M>(cs.R, a.R);
 
     FaceOut<M> fo;
-    int fl = face_stage<M>(g.mat, g.opt, s, Pa, Ua, a, P, U, cs, fo);
+    int fl = face_stage<M, F>(g.mat, g.opt, s, Pa, Ua, a, P, U, cs, fo);
 
     double Fr[NC];
 #pragma unroll
@@ -200,9 +200,9 @@ __device__ __forceinline__ void x_line(const GridView& g, double s, double r, in
     // fluxes and are not stored: the repair and the physical-state test skip them
     double Un[NC];
     bool bad;
-    update_cell<M>(g, r, U, fo.F, Fr, fo.ustar, usr, fo.zstar, zsr, Un, bad, store);
+    update_cell<M, F>(g, r, U, fo.F, Fr, fo.ustar, usr, fo.zstar, zsr, Un, bad, store);
     bool rep = false;
-    if (M == EULER && store) rep = euler_repair(g.mat, U, Un);
+    if (M == EULER && store) rep = euler_repair<F>(g.mat, U, Un);
 
     if (face_counted) { tl.slope += fl & 1; tl.evolve += (fl >> 1) & 1; }
     if (store) {
@@ -213,7 +213,7 @@ __device__ __forceinline__ void x_line(const GridView& g, double s, double r, in
         tl.bad += bad;
         if (LAST) {
             double su, sv;
-            wave_speeds(g.mat, Un, su, sv, (Model<M>*)nullptr);
+            wave_speeds<F>(g.mat, Un, su, sv, (Model<M>*)nullptr);
             tl.su = smax(tl.su, su);
             tl.sv = smax(tl.sv, sv);
         }
@@ -232,7 +232,7 @@ __device__ __forceinline__ void x_load(const GridView& g, int tx, int y, double*
 
 // One x-sweep tile: the calling wavefront covers columns [tx*60-2, tx*60+62) and rows
 // [row0, row0+nrows); it writes the middle 60 columns.
-template <int M, bool LAST>
+template <int M, bool LAST, bool F = false>
 __device__ __forceinline__ void x_tile(const GridView& g, double dt, int tx, int row0, int nrows, Tally& tl)
 {
     constexpr int NC = Model<M>::NC;
@@ -255,14 +255,14 @@ __device__ __forceinline__ void x_tile(const GridView& g, double dt, int tx, int
         } else {
             x_load<M>(g, tx, y, U);
         }
-        x_line<M, LAST>(g, s, r, tx, y, U, tl);
+        x_line<M, LAST, F>(g, s, r, tx, y, U, tl);
     }
 }
 
 // Godunov update of one cell: conservative update, then for Allaire the upwind z update and
 // the snapping physical-state test (A/sourcecode/allaire_diffuse_rowsweep.cpp:36-50 with
 // zupdate_upwind).
-template <int M>
+template <int M, bool F = false>
 __device__ __forceinline__ void update_cell_godunov(const GridView& g, double r, const double* U, double zL,
                                                     double zR, const double* FL, const double* FR, double usL,
                                                     double usR, double* Un, bool& bad, bool checked)
@@ -274,14 +274,14 @@ __device__ __forceinline__ void update_cell_godunov(const GridView& g, double r,
     if (M == ALLAIRE) {
         Un[5] = zupdate_upwind(r, zL, U[5], zR, usL, usR, g.opt.z_lo, g.opt.z_hi);
         snap(Un);
-        bad = checked && !allaire_physical_screened(Un);
+        bad = checked && !allaire_physical_screened<F>(Un);
     }
 }
 
 // One x-sweep tile of the first-order Godunov scheme (same tile geometry as x_tile): the face
 // left of each lane's cell is the Riemann problem of the left neighbour's and the cell's stored
 // states.
-template <int M, bool LAST>
+template <int M, bool LAST, bool F = false>
 __device__ __forceinline__ void x_tile_godunov(const GridView& g, double dt, int tx, int row0, int nrows, Tally& tl)
 {
     constexpr int NC = Model<M>::NC;
@@ -315,7 +315,7 @@ __device__ __forceinline__ void x_tile_godunov(const GridView& g, double dt, int
 #pragma unroll
         for (int k = 0; k < NC; ++k) Ua[k] = lane_from_left_or_self(U[k]);
         FaceOut<M> fo;
-        godunov_face<M>(g.mat, g.opt.wave_speeds, Ua, U, fo);
+        godunov_face<M, F>(g.mat, g.opt.wave_speeds, Ua, U, fo);
         double Fr[NC];
 #pragma unroll
         for (int k = 0; k < NC; ++k) Fr[k] = lane_from_right(fo.F[k]);
@@ -325,9 +325,9 @@ __device__ __forceinline__ void x_tile_godunov(const GridView& g, double dt, int
 
         double Un[NC];
         bool bad;
-        update_cell_godunov<M>(g, r, U, zL, zR, fo.F, Fr, fo.ustar, usr, Un, bad, store);
+        update_cell_godunov<M, F>(g, r, U, zL, zR, fo.F, Fr, fo.ustar, usr, Un, bad, store);
         bool rep = false;
-        if (M == EULER && store) rep = euler_repair(g.mat, U, Un);
+        if (M == EULER && store) rep = euler_repair<F>(g.mat, U, Un);
         if (store) {
             double* out = g.nxt + (long long)y * g.pitch + x;
 #pragma unroll
@@ -336,7 +336,7 @@ __device__ __forceinline__ void x_tile_godunov(const GridView& g, double dt, int
             tl.bad += bad;
             if (LAST) {
                 double su, sv;
-                wave_speeds(g.mat, Un, su, sv, (Model<M>*)nullptr);
+                wave_speeds<F>(g.mat, Un, su, sv, (Model<M>*)nullptr);
                 tl.su = smax(tl.su, su);
                 tl.sv = smax(tl.sv, sv);
             }
@@ -376,7 +376,7 @@ __device__ __forceinline__ void load_rot(const GridView& g, int y, int xs, doubl
 }
 
 // One y-sweep tile: each lane owns column x = bx*256 + lane and marches rows [y0, y1).
-template <int M, bool LAST>
+template <int M, bool LAST, bool F = false>
 __device__ __forceinline__ void y_tile(const GridView& g, double dt, int bx, int y0, int y1, Tally& tl)
 {
     constexpr int NC = Model<M>::NC;
@@ -393,12 +393,12 @@ __device__ __forceinline__ void y_tile(const GridView& g, double dt, int bx, int
     FaceOut<M> Fm;    // face c-1
 
     load_rot<M>(g, y0 - 2, xs, Um);
-    Pm.from_cons(g.mat, Um);
+    Pm.template from_cons<F>(g.mat, Um);
     load_rot<M>(g, y0 - 1, xs, U0);
-    P0.from_cons(g.mat, U0);
+    P0.template from_cons<F>(g.mat, U0);
     load_rot<M>(g, y0, xs, Up);
-    Pp.from_cons(g.mat, Up);
-    cell_stage<M>(g.mat, g.opt.limiter, s, P0, Pm.W, Pp.W, a);  // cell y0-1
+    Pp.template from_cons<F>(g.mat, Up);
+    cell_stage<M, F>(g.mat, g.opt.limiter, s, P0, Pm.W, Pp.W, a);  // cell y0-1
     // shift: c-1 <- y0-1, c <- y0
 #pragma unroll
     for (int k = 0; k < NC; ++k) { Um[k] = U0[k]; U0[k] = Up[k]; }
@@ -412,17 +412,17 @@ __device__ __forceinline__ void y_tile(const GridView& g, double dt, int bx, int
 #pragma unroll
         for (int k = 0; k < NC; ++k) Up[k] = Unext[k];
         if (c + 2 <= y1 + 1) load_rot<M>(g, c + 2, xs, Unext);
-        Pp.from_cons(g.mat, Up);
+        Pp.template from_cons<F>(g.mat, Up);
         CellStage<M> cs;
-        cell_stage<M>(g.mat, g.opt.limiter, s, P0, Pm.W, Pp.W, cs);
+        cell_stage<M, F>(g.mat, g.opt.limiter, s, P0, Pm.W, Pp.W, cs);
         FaceOut<M> fo;
-        int fl = face_stage<M>(g.mat, g.opt, s, Pm, Um, a, P0, U0, cs, fo);
+        int fl = face_stage<M, F>(g.mat, g.opt, s, Pm, Um, a, P0, U0, cs, fo);
         if (valid && (c < y1 || c == g.ny)) { tl.slope += fl & 1; tl.evolve += (fl >> 1) & 1; }
         if (c > y0) {
             // update cell c-1 from faces c-1 (Fm) and c (fo)
             double Un[NC];
             bool bad;
-            update_cell<M>(g, r, Um, Fm.F, fo.F, Fm.ustar, fo.ustar, Fm.zstar, fo.zstar, Un, bad);
+            update_cell<M, F>(g, r, Um, Fm.F, fo.F, Fm.ustar, fo.ustar, Fm.zstar, fo.zstar, Un, bad);
             // rotate back (m_x, m_y) -> (-m_y, m_x): E/sourcecode/euler_colsweep.cpp:163-168
             double t = Un[MX];
             Un[MX] = -Un[MY];
@@ -434,7 +434,7 @@ __device__ __forceinline__ void y_tile(const GridView& g, double dt, int bx, int
                 for (int k = 0; k < NC; ++k) old[k] = Um[k];
                 old[MX] = -Um[MY];
                 old[MY] = Um[MX];
-                rep = euler_repair(g.mat, old, Un);
+                rep = euler_repair<F>(g.mat, old, Un);
             }
             if (valid) {
                 double* out = g.nxt + (long long)(c - 1) * g.pitch + x;
@@ -444,7 +444,7 @@ __device__ __forceinline__ void y_tile(const GridView& g, double dt, int bx, int
                 tl.bad += bad;
                 if (LAST) {
                     double su, sv;
-                    wave_speeds(g.mat, Un, su, sv, (Model<M>*)nullptr);
+                    wave_speeds<F>(g.mat, Un, su, sv, (Model<M>*)nullptr);
                     tl.su = smax(tl.su, su);
                     tl.sv = smax(tl.sv, sv);
                 }
@@ -467,7 +467,7 @@ __device__ __forceinline__ void y_tile(const GridView& g, double dt, int bx, int
 // the columns -- lanes along y, neighbours by DPP -- and overwrites its columns in LDS with the
 // results; the workgroup (256 threads) then stores the middle 60 rows. Same values as y_tile
 // (same cell and face stages, rotated frame).
-template <int M, bool LAST, int COLS>
+template <int M, bool LAST, int COLS, bool F = false>
 __device__ __forceinline__ void y_tile_t(const GridView& g, double dt, int x0, int y0, double* sU, Tally& tl)
 {
     constexpr int NC = Model<M>::NC;
@@ -512,12 +512,12 @@ __device__ __forceinline__ void y_tile_t(const GridView& g, double dt, int x0, i
         U[MY] = -raw[MX];
 
         Prim<M> P;
-        P.from_cons(g.mat, U);
+        P.template from_cons<F>(g.mat, U);
         double Wm[NC], Wp[NC];
 #pragma unroll
         for (int k = 0; k < NC; ++k) { Wm[k] = lane_from_left_or_self(P.W[k]); Wp[k] = lane_from_right_or_self(P.W[k]); }
         CellStage<M> cs;
-        cell_stage<M>(g.mat, g.opt.limiter, s, P, Wm, Wp, cs);
+        cell_stage<M, F>(g.mat, g.opt.limiter, s, P, Wm, Wp, cs);
         Prim<M> Pa;
 #pragma unroll
         for (int k = 0; k < NC; ++k) Pa.W[k] = Wm[k];
@@ -529,7 +529,7 @@ __device__ __forceinline__ void y_tile_t(const GridView& g, double dt, int x0, i
         a.fR = lane_from_left_or_self_i(cs.fR ? 1 : 0) != 0;
         side_from_left<M>(cs.R, a.R);
         FaceOut<M> fo;
-        int fl = face_stage<M>(g.mat, g.opt, s, Pa, Ua, a, P, U, cs, fo);
+        int fl = face_stage<M, F>(g.mat, g.opt, s, Pa, Ua, a, P, U, cs, fo);
         double Fr[NC];
 #pragma unroll
         for (int k = 0; k < NC; ++k) Fr[k] = lane_from_right(fo.F[k]);
@@ -537,20 +537,20 @@ __device__ __forceinline__ void y_tile_t(const GridView& g, double dt, int x0, i
         double zsr = M == ALLAIRE ? lane_from_right(fo.zstar) : 0.0;
         double Un[NC];
         bool bad;
-        update_cell<M>(g, r, U, fo.F, Fr, fo.ustar, usr, fo.zstar, zsr, Un, bad, store);
+        update_cell<M, F>(g, r, U, fo.F, Fr, fo.ustar, usr, fo.zstar, zsr, Un, bad, store);
         // rotate back (m_x, m_y) -> (-m_y, m_x): E/sourcecode/euler_colsweep.cpp:163-168
         double t = Un[MX];
         Un[MX] = -Un[MY];
         Un[MY] = t;
         bool rep = false;
-        if (M == EULER && store) rep = euler_repair(g.mat, raw, Un);
+        if (M == EULER && store) rep = euler_repair<F>(g.mat, raw, Un);
         if (face_row && col_ok) { tl.slope += fl & 1; tl.evolve += (fl >> 1) & 1; }
         if (store) {
             tl.repair += rep;
             tl.bad += bad;
             if (LAST) {
                 double su, sv;
-                wave_speeds(g.mat, Un, su, sv, (Model<M>*)nullptr);
+                wave_speeds<F>(g.mat, Un, su, sv, (Model<M>*)nullptr);
                 tl.su = smax(tl.su, su);
                 tl.sv = smax(tl.sv, sv);
             }
@@ -580,7 +580,7 @@ __device__ __forceinline__ void y_tile_t(const GridView& g, double dt, int x0, i
 
 // One y-sweep tile of the first-order Godunov scheme: each lane marches rows [y0, y1) of its
 // column with a two-row window of stored states (faces y0 .. y1 need rows y0-1 .. y1).
-template <int M, bool LAST>
+template <int M, bool LAST, bool F = false>
 __device__ __forceinline__ void y_tile_godunov(const GridView& g, double dt, int bx, int y0, int y1, Tally& tl)
 {
     constexpr int NC = Model<M>::NC;
@@ -594,7 +594,7 @@ __device__ __forceinline__ void y_tile_godunov(const GridView& g, double dt, int
     load_rot<M>(g, y0 - 1, xs, Um);
     load_rot<M>(g, y0, xs, U0);
     FaceOut<M> Fm;
-    godunov_face<M>(g.mat, g.opt.wave_speeds, Um, U0, Fm);  // face y0
+    godunov_face<M, F>(g.mat, g.opt.wave_speeds, Um, U0, Fm);  // face y0
     load_rot<M>(g, y0 + 1, xs, Unext);
     for (int c = y0 + 1; c <= y1; ++c) {
         // window: Um = row c-2, U0 = row c-1, Up = row c
@@ -602,11 +602,11 @@ __device__ __forceinline__ void y_tile_godunov(const GridView& g, double dt, int
         for (int k = 0; k < NC; ++k) Up[k] = Unext[k];
         if (c + 1 <= y1) load_rot<M>(g, c + 1, xs, Unext);
         FaceOut<M> fo;
-        godunov_face<M>(g.mat, g.opt.wave_speeds, U0, Up, fo);  // face c
+        godunov_face<M, F>(g.mat, g.opt.wave_speeds, U0, Up, fo);  // face c
         // update row c-1 from faces c-1 (Fm) and c (fo)
         double Un[NC];
         bool bad;
-        update_cell_godunov<M>(g, r, U0, Um[NC - 1], Up[NC - 1], Fm.F, fo.F, Fm.ustar, fo.ustar, Un, bad, valid);
+        update_cell_godunov<M, F>(g, r, U0, Um[NC - 1], Up[NC - 1], Fm.F, fo.F, Fm.ustar, fo.ustar, Un, bad, valid);
         double t = Un[MX];
         Un[MX] = -Un[MY];
         Un[MY] = t;
@@ -617,7 +617,7 @@ __device__ __forceinline__ void y_tile_godunov(const GridView& g, double dt, int
             for (int k = 0; k < NC; ++k) old[k] = U0[k];
             old[MX] = -U0[MY];
             old[MY] = U0[MX];
-            rep = euler_repair(g.mat, old, Un);
+            rep = euler_repair<F>(g.mat, old, Un);
         }
         if (valid) {
             double* out = g.nxt + (long long)(c - 1) * g.pitch + x;
@@ -627,7 +627,7 @@ __device__ __forceinline__ void y_tile_godunov(const GridView& g, double dt, int
             tl.bad += bad;
             if (LAST) {
                 double su, sv;
-                wave_speeds(g.mat, Un, su, sv, (Model<M>*)nullptr);
+                wave_speeds<F>(g.mat, Un, su, sv, (Model<M>*)nullptr);
                 tl.su = smax(tl.su, su);
                 tl.sv = smax(tl.sv, sv);
             }

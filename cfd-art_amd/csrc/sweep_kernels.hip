@@ -20,7 +20,7 @@ namespace cfd {
 
 // Occupancy of the MUSCL-Hancock x sweeps, as measured fastest: Allaire at 3 wavefronts per SIMD
 // (at most 168 VGPRs), the Euler last sweep at 4 (at most 128 VGPRs instead of its natural 134).
-template <int M, bool LAST, bool GOD>
+template <int M, bool LAST, bool GOD, bool F>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M == ALLAIRE && !GOD ? 3 : (LAST && !GOD ? 4 : 1)))) void sweep_x_kernel(GridView g, double dt_arg, double dt_factor, int use_ctrl,
                                                      int xrows)
 {
@@ -37,12 +37,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M == ALLAIR
         return;
     }
     Tally tl;
-    if (GOD) x_tile_godunov<M, LAST>(g, dt, tx, row0, xrows, tl);
-    else x_tile<M, LAST>(g, dt, tx, row0, xrows, tl);
+    if (GOD) x_tile_godunov<M, LAST, F>(g, dt, tx, row0, xrows, tl);
+    else x_tile<M, LAST, F>(g, dt, tx, row0, xrows, tl);
     tl.flush(g.ctrl, LAST, blockIdx.x + blockIdx.y * gridDim.x);
 }
 
-template <int M, bool LAST, bool GOD>
+template <int M, bool LAST, bool GOD, bool F>
 __global__ __launch_bounds__(YBLOCK) void sweep_y_kernel(GridView g, double dt_arg, double dt_factor,
                                                         int use_ctrl, int H)
 {
@@ -59,14 +59,14 @@ __global__ __launch_bounds__(YBLOCK) void sweep_y_kernel(GridView g, double dt_a
         return;
     }
     Tally tl;
-    if (GOD) y_tile_godunov<M, LAST>(g, dt, bx, y0, y1, tl);
-    else y_tile<M, LAST>(g, dt, bx, y0, y1, tl);
+    if (GOD) y_tile_godunov<M, LAST, F>(g, dt, bx, y0, y1, tl);
+    else y_tile<M, LAST, F>(g, dt, bx, y0, y1, tl);
     tl.flush(g.ctrl, LAST, blockIdx.x + blockIdx.y * gridDim.x);
 }
 
 // Transposed y sweep (MUSCL-Hancock): a workgroup stages 64 rows x COLS columns through LDS and
 // its four wavefronts run lanes along y; tiles in column-major order like sweep_y_kernel.
-template <int M, bool LAST, int COLS>
+template <int M, bool LAST, int COLS, bool F>
 __global__ __launch_bounds__(256) void sweep_yt_kernel(GridView g, double dt_arg, double dt_factor, int use_ctrl)
 {
     __shared__ double sU[Model<M>::NC * 64 * (COLS + 1)];
@@ -83,7 +83,7 @@ __global__ __launch_bounds__(256) void sweep_yt_kernel(GridView g, double dt_arg
         return;
     }
     Tally tl;
-    y_tile_t<M, LAST, COLS>(g, dt, x0, y0, sU, tl);
+    y_tile_t<M, LAST, COLS, F>(g, dt, x0, y0, sU, tl);
     tl.flush(g.ctrl, LAST, b);
 }
 
@@ -223,46 +223,49 @@ hipError_t launch_set_time(Ctrl* c, double t, int n, hipStream_t st)
     return hipGetLastError();
 }
 
-template <int M, bool LAST, bool GOD>
+template <int M, bool LAST, bool GOD, bool F>
 static hipError_t launch_x(const GridView& g, double dt_arg, double f, int use_ctrl, int xrows, hipStream_t st)
 {
     dim3 grid((g.nx + XTILE - 1) / XTILE, (g.ny + 4 * xrows - 1) / (4 * xrows));
-    hipLaunchKernelGGL((sweep_x_kernel<M, LAST, GOD>), grid, dim3(256), 0, st, g, dt_arg, f, use_ctrl, xrows);
+    hipLaunchKernelGGL((sweep_x_kernel<M, LAST, GOD, F>), grid, dim3(256), 0, st, g, dt_arg, f, use_ctrl, xrows);
     return hipGetLastError();
 }
-template <int M, bool LAST, bool GOD>
+template <int M, bool LAST, bool GOD, bool F>
 static hipError_t launch_y(const GridView& g, double dt_arg, double f, int use_ctrl, int H, hipStream_t st)
 {
     if (y_transposed(H) && !GOD) {
         const int cols = yt_cols(H);
         dim3 grid((g.nx + cols - 1) / cols, (g.ny + YT_ROWS - 1) / YT_ROWS);
-        if (cols == 16) hipLaunchKernelGGL((sweep_yt_kernel<M, LAST, 16>), grid, dim3(256), 0, st, g, dt_arg, f, use_ctrl);
-        else hipLaunchKernelGGL((sweep_yt_kernel<M, LAST, 8>), grid, dim3(256), 0, st, g, dt_arg, f, use_ctrl);
+        if (cols == 16) hipLaunchKernelGGL((sweep_yt_kernel<M, LAST, 16, F>), grid, dim3(256), 0, st, g, dt_arg, f, use_ctrl);
+        else hipLaunchKernelGGL((sweep_yt_kernel<M, LAST, 8, F>), grid, dim3(256), 0, st, g, dt_arg, f, use_ctrl);
         return hipGetLastError();
     }
     if (H <= 0) H = 8;  // the first-order scheme has no transposed tile: 8-row strips (its fastest at 2048^2)
     dim3 grid((g.nx + YBLOCK - 1) / YBLOCK, (g.ny + H - 1) / H);
-    hipLaunchKernelGGL((sweep_y_kernel<M, LAST, GOD>), grid, dim3(YBLOCK), 0, st, g, dt_arg, f, use_ctrl, H);
+    hipLaunchKernelGGL((sweep_y_kernel<M, LAST, GOD, F>), grid, dim3(YBLOCK), 0, st, g, dt_arg, f, use_ctrl, H);
     return hipGetLastError();
 }
-template <int M, bool GOD>
+template <int M, bool GOD, bool F>
 static hipError_t launch_model(const GridView& g, int axis, bool last, double dt_arg, double dt_factor, int use_ctrl,
                                int yH, int xrows, hipStream_t st)
 {
-    if (axis == 0) return last ? launch_x<M, true, GOD>(g, dt_arg, dt_factor, use_ctrl, xrows, st)
-                               : launch_x<M, false, GOD>(g, dt_arg, dt_factor, use_ctrl, xrows, st);
-    return last ? launch_y<M, true, GOD>(g, dt_arg, dt_factor, use_ctrl, yH, st)
-                : launch_y<M, false, GOD>(g, dt_arg, dt_factor, use_ctrl, yH, st);
+    if (axis == 0) return last ? launch_x<M, true, GOD, F>(g, dt_arg, dt_factor, use_ctrl, xrows, st)
+                               : launch_x<M, false, GOD, F>(g, dt_arg, dt_factor, use_ctrl, xrows, st);
+    return last ? launch_y<M, true, GOD, F>(g, dt_arg, dt_factor, use_ctrl, yH, st)
+                : launch_y<M, false, GOD, F>(g, dt_arg, dt_factor, use_ctrl, yH, st);
 }
 
 hipError_t launch_sweep(const GridView& g, int model, int axis, bool last, double dt_arg,
                         double dt_factor, int use_ctrl, int yH, int xrows, hipStream_t st)
 {
-    if (model == EULER)
-        return g.opt.godunov ? launch_model<EULER, true>(g, axis, last, dt_arg, dt_factor, use_ctrl, yH, xrows, st)
-                             : launch_model<EULER, false>(g, axis, last, dt_arg, dt_factor, use_ctrl, yH, xrows, st);
-    return g.opt.godunov ? launch_model<ALLAIRE, true>(g, axis, last, dt_arg, dt_factor, use_ctrl, yH, xrows, st)
-                         : launch_model<ALLAIRE, false>(g, axis, last, dt_arg, dt_factor, use_ctrl, yH, xrows, st);
+#define CFD_LAUNCH(M, GOD, F) launch_model<M, GOD, F>(g, axis, last, dt_arg, dt_factor, use_ctrl, yH, xrows, st)
+    if (model == EULER) {
+        if (g.opt.godunov) return g.opt.fast ? CFD_LAUNCH(EULER, true, true) : CFD_LAUNCH(EULER, true, false);
+        return g.opt.fast ? CFD_LAUNCH(EULER, false, true) : CFD_LAUNCH(EULER, false, false);
+    }
+    if (g.opt.godunov) return g.opt.fast ? CFD_LAUNCH(ALLAIRE, true, true) : CFD_LAUNCH(ALLAIRE, true, false);
+    return g.opt.fast ? CFD_LAUNCH(ALLAIRE, false, true) : CFD_LAUNCH(ALLAIRE, false, false);
+#undef CFD_LAUNCH
 }
 
 int reduce_blocks(const GridView& g)

@@ -32,6 +32,7 @@ CONSTANTS = {
     "CFDART_NAIVE": cfdart.NAIVE, "CFDART_STRANG": cfdart.STRANG, "CFDART_ALTERNATING": cfdart.ALTERNATING,
     "CFDART_MUSCL_HANCOCK": cfdart.MUSCL_HANCOCK, "CFDART_GODUNOV": cfdart.GODUNOV,
     "CFDART_Y_TRANSPOSED": cfdart.Y_TRANSPOSED, "CFDART_Y_TRANSPOSED_NARROW": cfdart.Y_TRANSPOSED_NARROW,
+    "CFDART_ARITH_EXACT": cfdart.ARITH_EXACT, "CFDART_ARITH_FAST": cfdart.ARITH_FAST,
 }
 
 

@@ -1,0 +1,125 @@
+"""GPU accuracy of the fast arithmetic (cfdart_set_arith with CFDART_ARITH_FAST): divisions are
+the numerator times the divisor's reciprocal refined by one Newton step, square roots take one
+Goldschmidt step. The results are not bitwise the reference's arithmetic; the bar is per-field
+relative L1 and Linf within TOL = 1e-10 of the oracle on the same seeded inputs: euler_2048 and
+allaire_1024 after 100 alternating steps, the first-order Godunov scheme, every tile shape on a
+ragged grid and every splitting. Mass and energy stay conserved to round-off in a closed box, a
+free stream is preserved bitwise, and the exact mode on the same handle is bitwise again."""
+import math
+
+import numpy as np
+import pytest
+
+import cfdart
+import oracle
+from test_gpu_shapes import state
+from util import bitwise_equal, field_errors, interior
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+FAST = cfdart.ARITH_FAST
+
+
+def _within(a, b, label, ng=2):
+    errs = field_errors(interior(a, ng), interior(b, ng))
+    print(f"{label}: relL1/Linf per field={errs}", flush=True)
+    for l1, li in errs:
+        assert l1 <= TOL and li <= TOL, (label, errs)
+    return errs
+
+
+def _run_pair(d, h, splitting, steps, cfl, tile=None):
+    g = cfdart.Grid(d)
+    if tile:
+        g.set_tile(*tile)
+    g.set_arith(FAST)
+    g.upload(h)
+    tg, ng = g.run(splitting, steps, cfl=cfl)
+    ora = oracle.Sim(d, h)
+    no, _ = ora.run(splitting, steps, cfl=cfl)
+    assert ng == no == steps
+    assert abs(tg - ora.t) <= 1e-12 * ora.t, (tg, ora.t)
+    return g, ora
+
+
+def test_euler_2048_100_steps(gpu):
+    d, h = cfdart.configs.euler_2048()
+    print("case fast arithmetic, euler_2048, 100 alternating steps", flush=True)
+    g, ora = _run_pair(d, h, cfdart.ALTERNATING, 100, 0.8)
+    errs = _within(g.download(), ora.grid, "euler_2048 fast n=100")
+    assert max(l1 for l1, _ in errs) > 0.0  # the fast arithmetic is in use (not bitwise)
+    print(f"counters gpu {g.stats()} oracle {ora.counters()}", flush=True)
+
+
+def test_allaire_1024_100_steps(gpu):
+    d, h = cfdart.configs.allaire_1024()
+    print("case fast arithmetic, allaire_1024, 100 alternating steps", flush=True)
+    g, ora = _run_pair(d, h, cfdart.ALTERNATING, 100, 0.5)
+    _within(g.download(), ora.grid, "allaire_1024 fast n=100")
+    assert g.stats()[3] == 0  # no unphysical state
+
+
+@pytest.mark.parametrize("model", [0, 1])
+def test_godunov(gpu, model):
+    d, h = state(model, 200, 130, cfdart.GODUNOV, seed=11 + model)
+    print(f"case fast arithmetic, Godunov model={model}", flush=True)
+    g, ora = _run_pair(d, h, cfdart.ALTERNATING, 40, 0.8 if model == 0 else 0.5)
+    _within(g.download(), ora.grid, f"godunov model={model}", ng=d.ng)
+
+
+@pytest.mark.parametrize("model", [0, 1])
+@pytest.mark.parametrize("tile", [(cfdart.Y_TRANSPOSED, 4), (cfdart.Y_TRANSPOSED_NARROW, 2), (12, 3)])
+@pytest.mark.parametrize("splitting", [cfdart.NAIVE, cfdart.STRANG, cfdart.ALTERNATING])
+def test_tiles_and_splittings(gpu, model, tile, splitting):
+    d, h = state(model, 150, 97, cfdart.MUSCL_HANCOCK, seed=41 + model)
+    print(f"case fast arithmetic model={model} tile={tile} splitting={splitting}", flush=True)
+    g, ora = _run_pair(d, h, splitting, 12, 0.8 if model == 0 else 0.5, tile=tile)
+    _within(g.download(), ora.grid, f"model={model} tile={tile} splitting={splitting}")
+
+
+def test_conservation_closed_box(gpu):
+    d, h = cfdart.configs.euler_discs(1024, 32, (0.005, 0.03), (1, 1, 1, 1), seed=3)
+    print("case fast arithmetic, closed box 1024^2, 100 steps", flush=True)
+    g = cfdart.Grid(d)
+    g.set_arith(FAST)
+    g.upload(h)
+    g.run(cfdart.ALTERNATING, 100, cfl=0.8)
+    s = g.download()
+    assert g.stats()[2] == 0  # no positivity repair, which would not conserve
+    for k in (0, 3):
+        tot0 = math.fsum(interior(h)[..., k].ravel())
+        tot = math.fsum(interior(s)[..., k].ravel())
+        rel = abs(tot - tot0) / abs(tot0)
+        print(f"component {k}: relative change {rel:.3e}", flush=True)
+        assert rel <= 1e-13
+
+
+def test_free_stream(gpu):
+    d, _ = cfdart.configs.euler_discs(512, 4, (0.05, 0.1), (0, 0, 0, 0), seed=1)
+    h = np.empty((d.ny + 4, d.nx + 4, 4))
+    h[...] = (1.3, 1.3 * 0.7, 1.3 * -0.4, 2.5 / 0.4 + 0.5 * 1.3 * (0.49 + 0.16))
+    print("case fast arithmetic, free stream 512^2, 20 steps", flush=True)
+    g = cfdart.Grid(d)
+    g.set_arith(FAST)
+    g.upload(h)
+    g.run(cfdart.ALTERNATING, 20, cfl=0.8)
+    assert bitwise_equal(interior(g.download()), interior(h))
+
+
+def test_exact_again_on_the_same_handle(gpu):
+    d, h = state(0, 130, 70, cfdart.MUSCL_HANCOCK, seed=5)
+    print("case fast then exact arithmetic on one handle", flush=True)
+    g = cfdart.Grid(d)
+    g.set_arith(FAST)
+    g.upload(h)
+    g.run(cfdart.ALTERNATING, 3, cfl=0.8)
+    g.set_arith(cfdart.ARITH_EXACT)
+    g.upload(h)
+    t, n = g.run(cfdart.ALTERNATING, 6, cfl=0.8)
+    ora = oracle.Sim(d, h)
+    ora.run(cfdart.ALTERNATING, 6, cfl=0.8)
+    assert n == 6 and t == ora.t
+    assert bitwise_equal(interior(g.download()), interior(ora.grid))
+    assert g.stats() == ora.counters()
+    with pytest.raises(cfdart.CfdartError):
+        g.set_arith(2)
