@@ -127,7 +127,9 @@ def bench_batched(args):
         batch_shape.update(cfdart.autotune.tune_batch(tb, {model: ts}, steps=3, log=log))
         tb.close()
 
+    mode = cfdart.ARITH_FAST if args.arith == "fast" else cfdart.ARITH_EXACT
     b = cfdart.Batch(len(cases))
+    b.set_arith(mode)
     slots = []
     for d, h, cfl in cases:
         yh, xr = batch_shape[d.model]
@@ -146,6 +148,7 @@ def bench_batched(args):
         g = cfdart.Grid(d)
         yh, xr = cfdart.autotune.shape_for(table, d.model, d.nx, d.ny)
         g.set_tile(yh, xr)
+        g.set_arith(mode)
         g.upload(h)
         g.sync()
         t0 = time.perf_counter()
@@ -162,7 +165,7 @@ def bench_batched(args):
         "higher_is_better": True, "dtype": "f64", "data": "synthetic (seeded disc/bubble generators)",
         "config": {"workload": "batched", "grids": len(cases), "cells": sum(cells),
                    "euler": sum(1 for c in cases if c[0].model == 0), "sides": "log-uniform [128,1024]",
-                   "splitting": "alternating"},
+                   "splitting": "alternating", "arith": args.arith},
         "one_at_a_time": {"value": work / t_plain, "seconds": t_plain},
         "speedup_vs_one_at_a_time": t_plain / t_batch,
         "bitwise_equal_alone": f"{same}/{len(cases)}",
@@ -192,7 +195,8 @@ def bench_served(args):
 
     def opts(cap):
         return serve.opts(mem_budget=200 * 10**9, capacity=8192, chunk_steps=10, frame_every=50,
-                          frame_host_bytes=2 << 30, realtime=True, tile=tile, plain_tile=plain, live_cells=cap)
+                          frame_host_bytes=2 << 30, realtime=True, tile=tile, plain_tile=plain, live_cells=cap,
+                          arith=cfdart.ARITH_FAST if args.arith == "fast" else cfdart.ARITH_EXACT)
 
     o = opts(live_cap)
     rb, sb = serve.serve_batched(reqs, o)
@@ -220,7 +224,7 @@ def bench_served(args):
         "config": {"workload": "served", "requests": len(reqs), "rate_per_s": 50.0, "window_s": args.serve_window,
                    "euler_frac": 0.7, "sides": "log-uniform [64,2048]", "steps": "uniform [50,2000]",
                    "frame_every": 50, "mem_budget_bytes": 200 * 10**9, "live_cells": live_cap, "chunk_steps": 10,
-                   "tile": {"euler": tile[0], "allaire": tile[1]}},
+                   "tile": {"euler": tile[0], "allaire": tile[1]}, "arith": args.arith},
         "latency_s": {"batched": lb, "one_at_a_time": lo},
         "one_at_a_time": {"value": so.cell_updates / so.makespan, "makespan_s": so.makespan, "busy_s": busy},
         "makespan_s": sb.makespan,
@@ -246,7 +250,8 @@ def main():
     ap.add_argument("--tile", default="", help="Y_STRIP_ROWS,X_ROWS_PER_WAVE: fixed tile shape, no autotuning")
     ap.add_argument("--arith", default="fast", choices=["fast", "exact"],
                     help="single-grid workloads: fast = time the exact and the fast arithmetic and report the fast "
-                         "one with its accuracy against the exact state; exact = the exact arithmetic only")
+                         "one with its accuracy against the exact state; exact = the exact arithmetic only. Batched "
+                         "and served workloads run the batch and its one-at-a-time baseline in the given arithmetic")
     ap.add_argument("--workload", default="euler_2048",
                     choices=["euler_2048", "allaire_1024", "euler_200", "euler_slab", "godunov_2048", "batched",
                              "served"])

@@ -115,6 +115,7 @@ def lib():
         L.cfdart_batch_stats.argtypes = [C.c_void_p, C.c_int, P(C.c_longlong)]
         L.cfdart_batch_sync.argtypes = [C.c_void_p]
         L.cfdart_batch_set_tile.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+        L.cfdart_batch_set_arith.argtypes = [C.c_void_p, C.c_int]
         L.cfdart_batch_info.argtypes = [C.c_void_p, P(C.c_int), P(C.c_size_t), P(C.c_int)]
         L.cfdart_batch_frame.argtypes = [C.c_void_p, C.c_int, dp]
         L.cfdart_batch_frames_done.argtypes = [C.c_void_p]
@@ -399,6 +400,10 @@ class Batch:
 
     def set_tile(self, slot, y_strip_rows, x_rows_per_wave):
         _check(lib().cfdart_batch_set_tile(self.h, slot, y_strip_rows, x_rows_per_wave))
+
+    def set_arith(self, mode):
+        """ARITH_EXACT (default) or ARITH_FAST for every grid of the batch, from the next step."""
+        _check(lib().cfdart_batch_set_arith(self.h, mode))
 
     def info(self):
         live = C.c_int()

@@ -81,7 +81,7 @@ __device__ __forceinline__ int batch_axis(const BatchGrid* bg, int phase)
 
 // x sweep of one phase over the x tiles of every live grid of model M (one tile per workgroup,
 // the tile of sweep_x_kernel). Tiles of grids on the other axis this phase exit at once.
-template <int M, bool LAST>
+template <int M, bool LAST, bool F>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M == ALLAIRE ? 3 : (LAST ? 4 : 1)))) void
 batch_x_kernel(BatchGrid* grids, const Tile* tiles, const BatchQueue* q, int phase)
 {
@@ -93,13 +93,13 @@ batch_x_kernel(BatchGrid* grids, const Tile* tiles, const BatchQueue* q, int pha
     const GridView g = batch_view(bg, phase);
     const int xrows = bg->xrows;
     Tally ta;
-    x_tile<M, LAST>(g, bg->ctrl.dt, tl.a, (tl.b * 4 + (threadIdx.x >> 6)) * xrows, xrows, ta);
+    x_tile<M, LAST, F>(g, bg->ctrl.dt, tl.a, (tl.b * 4 + (threadIdx.x >> 6)) * xrows, xrows, ta);
     ta.flush(&bg->ctrl, LAST, t);
 }
 
 // y sweep of one phase over the y tiles of every live grid of model M: the transposed tile of
 // sweep_yt_kernel (64 rows x 16 columns through LDS, lanes along y).
-template <int M, bool LAST>
+template <int M, bool LAST, bool F>
 __global__ __launch_bounds__(256) void batch_y_kernel(BatchGrid* grids, const Tile* tiles, const BatchQueue* q,
                                                       int phase)
 {
@@ -111,7 +111,7 @@ __global__ __launch_bounds__(256) void batch_y_kernel(BatchGrid* grids, const Ti
     if (batch_axis(bg, phase) != 1) return;
     const GridView g = batch_view(bg, phase);
     Tally ta;
-    y_tile_t<M, LAST, BATCH_YT_COLS>(g, bg->ctrl.dt, tl.a * BATCH_YT_COLS, tl.b * YT_ROWS, sU, ta);
+    y_tile_t<M, LAST, BATCH_YT_COLS, F>(g, bg->ctrl.dt, tl.a * BATCH_YT_COLS, tl.b * YT_ROWS, sU, ta);
     ta.flush(&bg->ctrl, LAST, t);
 }
 
@@ -184,6 +184,7 @@ struct cfdart_batch {
     hipEvent_t fork_ev, join_ev;
     hipGraphExec_t exec;
     bool have_graph;
+    int arith;  // CFDART_ARITH_EXACT or CFDART_ARITH_FAST, for every grid of the batch
     int ntiles;
     std::multimap<size_t, double*> pool;  // freed state buffers for reuse, by size
     int* d_status;           // per slot: steps taken, flags (1 running, 2 error)
@@ -278,6 +279,17 @@ int rebuild_tiles(cfdart_batch* b)
     return 0;
 }
 
+// The x and y kernels of model M for both sweep phases of a step, on one stream (each kernel's
+// grid is its work-list's capacity; items past the live count exit at once).
+template <int M, bool F> void launch_phases(cfdart_batch* b, hipStream_t st)
+{
+    const dim3 bx((unsigned)b->tile_cap[M][0]), by((unsigned)b->tile_cap[M][1]);
+    hipLaunchKernelGGL((cfd::batch_x_kernel<M, false, F>), bx, dim3(256), 0, st, b->d_grids, b->d_tiles[M][0], b->d_q, 0);
+    hipLaunchKernelGGL((cfd::batch_y_kernel<M, false, F>), by, dim3(256), 0, st, b->d_grids, b->d_tiles[M][1], b->d_q, 0);
+    hipLaunchKernelGGL((cfd::batch_x_kernel<M, true, F>), bx, dim3(256), 0, st, b->d_grids, b->d_tiles[M][0], b->d_q, 1);
+    hipLaunchKernelGGL((cfd::batch_y_kernel<M, true, F>), by, dim3(256), 0, st, b->d_grids, b->d_tiles[M][1], b->d_q, 1);
+}
+
 // One step = both sweep phases of the Euler and the Allaire kernels (two concurrent branches),
 // joined before the per-grid loop bookkeeping.
 int capture_graph(cfdart_batch* b)
@@ -292,21 +304,19 @@ int capture_graph(cfdart_batch* b)
                 b->tile_cap[m][ax] = 1;
                 BCHK(hipMalloc(&b->d_tiles[m][ax], sizeof(Tile)));
             }
-    auto nb = [&](int m, int ax) { return dim3((unsigned)b->tile_cap[m][ax]); };
     hipGraph_t graph;
     BCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
     BCHK(hipEventRecord(b->fork_ev, b->st));
     BCHK(hipStreamWaitEvent(b->st2, b->fork_ev, 0));
     using cfd::ALLAIRE;
     using cfd::EULER;
-    hipLaunchKernelGGL((cfd::batch_x_kernel<EULER, false>), nb(0, 0), dim3(256), 0, b->st, b->d_grids, b->d_tiles[0][0], b->d_q, 0);
-    hipLaunchKernelGGL((cfd::batch_y_kernel<EULER, false>), nb(0, 1), dim3(256), 0, b->st, b->d_grids, b->d_tiles[0][1], b->d_q, 0);
-    hipLaunchKernelGGL((cfd::batch_x_kernel<EULER, true>), nb(0, 0), dim3(256), 0, b->st, b->d_grids, b->d_tiles[0][0], b->d_q, 1);
-    hipLaunchKernelGGL((cfd::batch_y_kernel<EULER, true>), nb(0, 1), dim3(256), 0, b->st, b->d_grids, b->d_tiles[0][1], b->d_q, 1);
-    hipLaunchKernelGGL((cfd::batch_x_kernel<ALLAIRE, false>), nb(1, 0), dim3(256), 0, b->st2, b->d_grids, b->d_tiles[1][0], b->d_q, 0);
-    hipLaunchKernelGGL((cfd::batch_y_kernel<ALLAIRE, false>), nb(1, 1), dim3(256), 0, b->st2, b->d_grids, b->d_tiles[1][1], b->d_q, 0);
-    hipLaunchKernelGGL((cfd::batch_x_kernel<ALLAIRE, true>), nb(1, 0), dim3(256), 0, b->st2, b->d_grids, b->d_tiles[1][0], b->d_q, 1);
-    hipLaunchKernelGGL((cfd::batch_y_kernel<ALLAIRE, true>), nb(1, 1), dim3(256), 0, b->st2, b->d_grids, b->d_tiles[1][1], b->d_q, 1);
+    if (b->arith == CFDART_ARITH_FAST) {
+        launch_phases<EULER, true>(b, b->st);
+        launch_phases<ALLAIRE, true>(b, b->st2);
+    } else {
+        launch_phases<EULER, false>(b, b->st);
+        launch_phases<ALLAIRE, false>(b, b->st2);
+    }
     BCHK(hipEventRecord(b->join_ev, b->st2));
     BCHK(hipStreamWaitEvent(b->st, b->join_ev, 0));
     hipLaunchKernelGGL(cfd::batch_finalize_kernel, dim3((b->cap + 255) / 256), dim3(256), 0, b->st, b->d_grids,
@@ -349,6 +359,7 @@ int cfdart_batch_create(int capacity, size_t mem_budget_bytes, cfdart_batch** ou
             b->ntile[m][ax] = 0;
         }
     b->have_graph = false;
+    b->arith = CFDART_ARITH_EXACT;
     b->ntiles = 0;
     b->d_status = nullptr;
     b->tiles_dirty = false;
@@ -522,6 +533,7 @@ int admit(cfdart_batch* b, const cfdart_desc* d, const double* host, const cfdar
     for (int i = 0; i < 4; ++i) v.bc[i] = g.bc[i];
     v.mat = g.mat;
     v.opt = g.opt;
+    v.opt.fast = b->arith == CFDART_ARITH_FAST ? 1 : 0;  // the first dt in the batch's arithmetic
     v.dx = g.dx;
     v.dy = g.dy;
     v.dx_half_col = g.dx_half_col;
@@ -676,6 +688,19 @@ int cfdart_batch_set_tile(cfdart_batch* b, int slot, int yH, int xrows)
     BCHK(hipMemcpyAsync(&(b->d_grids + slot)->yH, &g.yH, 2 * sizeof(int), hipMemcpyHostToDevice, b->st));
     BCHK(hipStreamSynchronize(b->st));
     b->tiles_dirty = true;
+    return 0;
+}
+
+int cfdart_batch_set_arith(cfdart_batch* b, int mode)
+{
+    if (!b || (mode != CFDART_ARITH_EXACT && mode != CFDART_ARITH_FAST)) return bfail(CFDART_E_ARG, "bad batch or arithmetic mode");
+    if (mode == b->arith) return 0;
+    b->arith = mode;
+    if (b->have_graph) {  // the captured step holds the other mode's kernels
+        BCHK(hipStreamSynchronize(b->st));
+        (void)hipGraphExecDestroy(b->exec);
+        b->have_graph = false;
+    }
     return 0;
 }
 

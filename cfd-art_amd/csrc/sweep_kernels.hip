@@ -91,7 +91,9 @@ __global__ __launch_bounds__(256) void sweep_yt_kernel(GridView g, double dt_arg
 
 // compute_dt's per-cell pass over the current state (E/sourcecode/euler_timestep.cpp:12-30):
 // maxima into the control block, per-block mass partial sums (fixed-shape tree, deterministic).
-template <int M>
+// The wave speeds take the grid's arithmetic, as the last sweep of a step computes them, so that a
+// run's result does not depend on how it is split into cfdart_run calls.
+template <int M, bool F>
 __global__ __launch_bounds__(256) void reduce_dt_kernel(GridView g, double* partials)
 {
     constexpr int NC = Model<M>::NC;
@@ -106,7 +108,7 @@ __global__ __launch_bounds__(256) void reduce_dt_kernel(GridView g, double* part
 #pragma unroll
         for (int k = 0; k < NC; ++k) U[k] = b[k * g.plane];
         double a, c;
-        wave_speeds(g.mat, U, a, c, (Model<M>*)nullptr);
+        wave_speeds<F>(g.mat, U, a, c, (Model<M>*)nullptr);
         su = smax(su, a);
         sv = smax(sv, c);
         m1 += U[0];
@@ -278,8 +280,13 @@ int reduce_blocks(const GridView& g)
 hipError_t launch_reduce_dt(const GridView& g, int model, double* partials, hipStream_t st)
 {
     int nb = reduce_blocks(g);
-    if (model == EULER) hipLaunchKernelGGL(reduce_dt_kernel<EULER>, dim3(nb), dim3(256), 0, st, g, partials);
-    else hipLaunchKernelGGL(reduce_dt_kernel<ALLAIRE>, dim3(nb), dim3(256), 0, st, g, partials);
+    if (model == EULER) {
+        if (g.opt.fast) hipLaunchKernelGGL((reduce_dt_kernel<EULER, true>), dim3(nb), dim3(256), 0, st, g, partials);
+        else hipLaunchKernelGGL((reduce_dt_kernel<EULER, false>), dim3(nb), dim3(256), 0, st, g, partials);
+    } else {
+        if (g.opt.fast) hipLaunchKernelGGL((reduce_dt_kernel<ALLAIRE, true>), dim3(nb), dim3(256), 0, st, g, partials);
+        else hipLaunchKernelGGL((reduce_dt_kernel<ALLAIRE, false>), dim3(nb), dim3(256), 0, st, g, partials);
+    }
     return hipGetLastError();
 }
 

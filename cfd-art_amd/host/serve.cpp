@@ -163,6 +163,10 @@ int cfdart_serve_batched(const cfdart_request* reqs, int n, const cfdart_serve_o
     cfdart_batch* b = nullptr;
     int rc = cfdart_batch_create(o->capacity, o->mem_budget, &b);
     if (rc) return rc;
+    if ((rc = cfdart_batch_set_arith(b, o->arith)) != 0) {
+        cfdart_batch_destroy(b);
+        return rc;
+    }
     FrameRing ring(o->frame_every > 0 ? o->frame_host_bytes : 0);
     std::vector<int> owner(o->capacity, -1), steps(o->capacity), flags(o->capacity);
     std::deque<int> waiting;
@@ -296,6 +300,7 @@ int cfdart_serve_one_at_a_time(const cfdart_request* reqs, int n, const cfdart_s
             continue;
         }
         rc = d.model == CFDART_EULER ? cfdart_init_random_discs(g, &dp) : cfdart_init_shocked_bubbles(g, &bp);
+        if (!rc) rc = cfdart_set_arith(g, o->arith);
         const int m = d.model;
         if (!rc && (o->plain_y_strip_rows[m] != 0 || o->plain_x_rows_per_wave[m] != 0))
             rc = cfdart_set_tile(g, o->plain_y_strip_rows[m] != 0 ? o->plain_y_strip_rows[m] : CFDART_Y_TRANSPOSED,

@@ -72,6 +72,9 @@ int cfdart_batch_sync(cfdart_batch* b);
 /* Changes a grid's tile shape (used by the autotuner between steps); y_strip_rows as in
  * cfdart_batch_add. */
 int cfdart_batch_set_tile(cfdart_batch* b, int slot, int y_strip_rows, int x_rows_per_wave);
+/* Arithmetic of every grid of the batch, as cfdart_set_arith (default CFDART_ARITH_EXACT); takes
+ * effect from the next step. No reference counterpart. */
+int cfdart_batch_set_arith(cfdart_batch* b, int mode);
 /* Live grids, device bytes held by grid state, tiles in the work-list (each may be NULL; asking
  * for the tile count rebuilds a work-list left stale by admissions or removals). */
 int cfdart_batch_info(cfdart_batch* b, int* live, size_t* bytes, int* tiles);

@@ -68,6 +68,7 @@ typedef struct cfdart_serve_opts {
     long long live_cells;    /* batched: admit while the live grids hold at most this many cells
                                 (enough to fill the GPU; the rest wait first come first served,
                                 which shortens each admitted request's latency); 0 = no limit */
+    int arith;               /* CFDART_ARITH_EXACT (0) or CFDART_ARITH_FAST, batched and one at a time */
 } cfdart_serve_opts;
 
 typedef struct cfdart_request_result {
