@@ -19,101 +19,14 @@
 #include <string>
 #include <vector>
 
+#include "batch_kernels.h"
 #include "ic_device.h"
-#include "sweep_device.h"
 
 namespace cfd {
 void set_last_error(const std::string& msg);  // cfdart_api.hip
 }
 
 namespace cfd {
-
-struct BatchGrid {
-    double* buf[2];
-    long long plane;
-    int pitch, nx, ny;
-    int model, splitting;
-    int bc[4];
-    Material mat;
-    SchemeOpts opt;
-    double dx, dy, dx_half_col;
-    int yH, xrows;
-    int steps_left;
-    int live;
-    Ctrl ctrl;
-};
-
-struct Tile {
-    int grid, axis, a, b;  // x: (column tile, row group); y: (column block, row block)
-};
-
-// Work-list lengths, per model and axis (each list has its own kernel; the launch covers the
-// list's capacity and workgroups past its length exit at once).
-struct BatchQueue {
-    int ntiles[2][2];  // [model][axis]
-};
-
-__device__ __forceinline__ GridView batch_view(BatchGrid* bg, int phase)
-{
-    GridView g;
-    g.cur = bg->buf[phase];
-    g.nxt = bg->buf[1 - phase];
-    g.plane = bg->plane;
-    g.pitch = bg->pitch;
-    g.nx = bg->nx;
-    g.ny = bg->ny;
-    for (int i = 0; i < 4; ++i) g.bc[i] = bg->bc[i];
-    g.mat = bg->mat;
-    g.opt = bg->opt;
-    g.dx = bg->dx;
-    g.dy = bg->dy;
-    g.dx_half_col = bg->dx_half_col;
-    g.ctrl = &bg->ctrl;
-    return g;
-}
-
-// The sweep axis of a grid in a phase, or -1 when the grid does not run (dead or finished).
-__device__ __forceinline__ int batch_axis(const BatchGrid* bg, int phase)
-{
-    if (!bg->live || !bg->ctrl.active) return -1;
-    return (bg->splitting == 2 && (bg->ctrl.n & 1)) ? 1 - phase : phase;
-}
-
-// x sweep of one phase over the x tiles of every live grid of model M (one tile per workgroup,
-// the tile of sweep_x_kernel). Tiles of grids on the other axis this phase exit at once.
-template <int M, bool LAST, bool F>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M == ALLAIRE ? 3 : (LAST ? 4 : 1)))) void
-batch_x_kernel(BatchGrid* grids, const Tile* tiles, const BatchQueue* q, int phase)
-{
-    const int t = blockIdx.x;
-    if (t >= q->ntiles[M][0]) return;
-    const Tile tl = tiles[t];
-    BatchGrid* bg = grids + tl.grid;
-    if (batch_axis(bg, phase) != 0) return;
-    const GridView g = batch_view(bg, phase);
-    const int xrows = bg->xrows;
-    Tally ta;
-    x_tile<M, LAST, F>(g, bg->ctrl.dt, tl.a, (tl.b * 4 + (threadIdx.x >> 6)) * xrows, xrows, ta);
-    ta.flush(&bg->ctrl, LAST, t);
-}
-
-// y sweep of one phase over the y tiles of every live grid of model M: the transposed tile of
-// sweep_yt_kernel (64 rows x 16 columns through LDS, lanes along y).
-template <int M, bool LAST, bool F>
-__global__ __launch_bounds__(256) void batch_y_kernel(BatchGrid* grids, const Tile* tiles, const BatchQueue* q,
-                                                      int phase)
-{
-    __shared__ double sU[Model<M>::NC * 64 * (BATCH_YT_COLS + 1)];
-    const int t = blockIdx.x;
-    if (t >= q->ntiles[M][1]) return;
-    const Tile tl = tiles[t];
-    BatchGrid* bg = grids + tl.grid;
-    if (batch_axis(bg, phase) != 1) return;
-    const GridView g = batch_view(bg, phase);
-    Tally ta;
-    y_tile_t<M, LAST, BATCH_YT_COLS, F>(g, bg->ctrl.dt, tl.a * BATCH_YT_COLS, tl.b * YT_ROWS, sU, ta);
-    ta.flush(&bg->ctrl, LAST, t);
-}
 
 // End of a step for every live grid (E/headers/simulation.hpp:56-58 per grid, plus the
 // step budget).
@@ -279,17 +192,6 @@ int rebuild_tiles(cfdart_batch* b)
     return 0;
 }
 
-// The x and y kernels of model M for both sweep phases of a step, on one stream (each kernel's
-// grid is its work-list's capacity; items past the live count exit at once).
-template <int M, bool F> void launch_phases(cfdart_batch* b, hipStream_t st)
-{
-    const dim3 bx((unsigned)b->tile_cap[M][0]), by((unsigned)b->tile_cap[M][1]);
-    hipLaunchKernelGGL((cfd::batch_x_kernel<M, false, F>), bx, dim3(256), 0, st, b->d_grids, b->d_tiles[M][0], b->d_q, 0);
-    hipLaunchKernelGGL((cfd::batch_y_kernel<M, false, F>), by, dim3(256), 0, st, b->d_grids, b->d_tiles[M][1], b->d_q, 0);
-    hipLaunchKernelGGL((cfd::batch_x_kernel<M, true, F>), bx, dim3(256), 0, st, b->d_grids, b->d_tiles[M][0], b->d_q, 1);
-    hipLaunchKernelGGL((cfd::batch_y_kernel<M, true, F>), by, dim3(256), 0, st, b->d_grids, b->d_tiles[M][1], b->d_q, 1);
-}
-
 // One step = both sweep phases of the Euler and the Allaire kernels (two concurrent branches),
 // joined before the per-grid loop bookkeeping.
 int capture_graph(cfdart_batch* b)
@@ -311,11 +213,11 @@ int capture_graph(cfdart_batch* b)
     using cfd::ALLAIRE;
     using cfd::EULER;
     if (b->arith == CFDART_ARITH_FAST) {
-        launch_phases<EULER, true>(b, b->st);
-        launch_phases<ALLAIRE, true>(b, b->st2);
+        cfd::launch_batch_phases_fast(EULER, b->d_grids, b->d_tiles[EULER], b->tile_cap[EULER], b->d_q, b->st);
+        cfd::launch_batch_phases_fast(ALLAIRE, b->d_grids, b->d_tiles[ALLAIRE], b->tile_cap[ALLAIRE], b->d_q, b->st2);
     } else {
-        launch_phases<EULER, false>(b, b->st);
-        launch_phases<ALLAIRE, false>(b, b->st2);
+        cfd::launch_batch_phases<EULER, false>(b->d_grids, b->d_tiles[EULER], b->tile_cap[EULER], b->d_q, b->st);
+        cfd::launch_batch_phases<ALLAIRE, false>(b->d_grids, b->d_tiles[ALLAIRE], b->tile_cap[ALLAIRE], b->d_q, b->st2);
     }
     BCHK(hipEventRecord(b->join_ev, b->st2));
     BCHK(hipStreamWaitEvent(b->st, b->join_ev, 0));

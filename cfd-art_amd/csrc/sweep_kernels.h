@@ -54,6 +54,11 @@ constexpr int BATCH_YT_COLS = 16;  // the batch's y tile: the 16-column transpos
 hipError_t launch_sweep(const GridView& g, int model, int axis, bool last, double dt_arg,
                         double dt_factor, int use_ctrl, int yH, int xrows, hipStream_t st);
 hipError_t launch_reduce_dt(const GridView& g, int model, double* partials, hipStream_t st);
+// The fast arithmetic's launchers (sweep_kernels_fast.hip); launch_sweep and launch_reduce_dt
+// forward to them for a grid whose opt.fast is set.
+hipError_t launch_sweep_fast(const GridView& g, int model, int axis, bool last, double dt_arg, double dt_factor,
+                             int use_ctrl, int yH, int xrows, hipStream_t st);
+hipError_t launch_reduce_dt_fast(const GridView& g, int model, double* partials, hipStream_t st);
 hipError_t launch_init_dt(Ctrl* c, double dx, double dy, hipStream_t st);
 // Mass history entry of the current state (t, mass1, mass2 into hist[0..2]) when the time loop
 // is running: the sums compute_dt records, from the same per-block partials.

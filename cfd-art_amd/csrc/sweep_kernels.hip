@@ -12,9 +12,19 @@
 // face fluxes, the conservative update, the Euler repair / Allaire z-update and snapping, and,
 // in the last sweep of a step, the per-cell wave speeds of compute_dt
 // (E/sourcecode/euler_timestep.cpp:14-25) reduced to one atomic max per wavefront.
+//
+// This file is compiled twice. As itself it builds the exact arithmetic (F = false, no FP
+// contraction: bitwise the reference's arithmetic) and everything that has one arithmetic only
+// (bookkeeping, mass, the launchers' dispatch). Through sweep_kernels_fast.hip (CFD_FAST_TU = 1)
+// it builds the same kernel templates for the fast arithmetic (F = true) with FMA contraction
+// allowed, and that arithmetic's launchers.
 #include "sweep_device.h"
 
 #include <hip/hip_runtime.h>
+
+#ifndef CFD_FAST_TU
+#define CFD_FAST_TU 0
+#endif
 
 namespace cfd {
 
@@ -132,6 +142,7 @@ __global__ __launch_bounds__(256) void reduce_dt_kernel(GridView g, double* part
     }
 }
 
+#if !CFD_FAST_TU
 // The mass sums of compute_dt (E/sourcecode/euler_timestep.cpp:26,32-33): the same per-block
 // partials as reduce_dt_kernel (same decomposition, same order), reading the density planes only.
 template <int M>
@@ -225,6 +236,8 @@ hipError_t launch_set_time(Ctrl* c, double t, int n, hipStream_t st)
     return hipGetLastError();
 }
 
+#endif  // !CFD_FAST_TU
+
 template <int M, bool LAST, bool GOD, bool F>
 static hipError_t launch_x(const GridView& g, double dt_arg, double f, int use_ctrl, int xrows, hipStream_t st)
 {
@@ -257,17 +270,46 @@ static hipError_t launch_model(const GridView& g, int axis, bool last, double dt
                 : launch_y<M, false, GOD, F>(g, dt_arg, dt_factor, use_ctrl, yH, st);
 }
 
+template <bool F>
+static hipError_t launch_sweep_t(const GridView& g, int model, int axis, bool last, double dt_arg, double dt_factor,
+                                 int use_ctrl, int yH, int xrows, hipStream_t st)
+{
+#define CFD_LAUNCH(M, GOD) launch_model<M, GOD, F>(g, axis, last, dt_arg, dt_factor, use_ctrl, yH, xrows, st)
+    if (model == EULER) return g.opt.godunov ? CFD_LAUNCH(EULER, true) : CFD_LAUNCH(EULER, false);
+    return g.opt.godunov ? CFD_LAUNCH(ALLAIRE, true) : CFD_LAUNCH(ALLAIRE, false);
+#undef CFD_LAUNCH
+}
+
+template <bool F>
+static hipError_t launch_reduce_dt_t(const GridView& g, int model, double* partials, hipStream_t st)
+{
+    const int nb = reduce_blocks(g);
+    if (model == EULER) hipLaunchKernelGGL((reduce_dt_kernel<EULER, F>), dim3(nb), dim3(256), 0, st, g, partials);
+    else hipLaunchKernelGGL((reduce_dt_kernel<ALLAIRE, F>), dim3(nb), dim3(256), 0, st, g, partials);
+    return hipGetLastError();
+}
+
+#if CFD_FAST_TU
+hipError_t launch_sweep_fast(const GridView& g, int model, int axis, bool last, double dt_arg, double dt_factor,
+                             int use_ctrl, int yH, int xrows, hipStream_t st)
+{
+    return launch_sweep_t<true>(g, model, axis, last, dt_arg, dt_factor, use_ctrl, yH, xrows, st);
+}
+hipError_t launch_reduce_dt_fast(const GridView& g, int model, double* partials, hipStream_t st)
+{
+    return launch_reduce_dt_t<true>(g, model, partials, st);
+}
+#else
 hipError_t launch_sweep(const GridView& g, int model, int axis, bool last, double dt_arg,
                         double dt_factor, int use_ctrl, int yH, int xrows, hipStream_t st)
 {
-#define CFD_LAUNCH(M, GOD, F) launch_model<M, GOD, F>(g, axis, last, dt_arg, dt_factor, use_ctrl, yH, xrows, st)
-    if (model == EULER) {
-        if (g.opt.godunov) return g.opt.fast ? CFD_LAUNCH(EULER, true, true) : CFD_LAUNCH(EULER, true, false);
-        return g.opt.fast ? CFD_LAUNCH(EULER, false, true) : CFD_LAUNCH(EULER, false, false);
-    }
-    if (g.opt.godunov) return g.opt.fast ? CFD_LAUNCH(ALLAIRE, true, true) : CFD_LAUNCH(ALLAIRE, true, false);
-    return g.opt.fast ? CFD_LAUNCH(ALLAIRE, false, true) : CFD_LAUNCH(ALLAIRE, false, false);
-#undef CFD_LAUNCH
+    if (g.opt.fast) return launch_sweep_fast(g, model, axis, last, dt_arg, dt_factor, use_ctrl, yH, xrows, st);
+    return launch_sweep_t<false>(g, model, axis, last, dt_arg, dt_factor, use_ctrl, yH, xrows, st);
+}
+hipError_t launch_reduce_dt(const GridView& g, int model, double* partials, hipStream_t st)
+{
+    if (g.opt.fast) return launch_reduce_dt_fast(g, model, partials, st);
+    return launch_reduce_dt_t<false>(g, model, partials, st);
 }
 
 int reduce_blocks(const GridView& g)
@@ -275,19 +317,6 @@ int reduce_blocks(const GridView& g)
     long long n = (long long)g.nx * g.ny;
     long long b = (n + 255) / 256;
     return (int)(b < 1024 ? b : 1024);
-}
-
-hipError_t launch_reduce_dt(const GridView& g, int model, double* partials, hipStream_t st)
-{
-    int nb = reduce_blocks(g);
-    if (model == EULER) {
-        if (g.opt.fast) hipLaunchKernelGGL((reduce_dt_kernel<EULER, true>), dim3(nb), dim3(256), 0, st, g, partials);
-        else hipLaunchKernelGGL((reduce_dt_kernel<EULER, false>), dim3(nb), dim3(256), 0, st, g, partials);
-    } else {
-        if (g.opt.fast) hipLaunchKernelGGL((reduce_dt_kernel<ALLAIRE, true>), dim3(nb), dim3(256), 0, st, g, partials);
-        else hipLaunchKernelGGL((reduce_dt_kernel<ALLAIRE, false>), dim3(nb), dim3(256), 0, st, g, partials);
-    }
-    return hipGetLastError();
 }
 
 hipError_t launch_mass_record(const GridView& g, int model, double* partials, double* hist, hipStream_t st)
@@ -312,5 +341,7 @@ hipError_t launch_finalize(Ctrl* c, double dx, double dy, hipStream_t st)
     hipLaunchKernelGGL(finalize_step_kernel, dim3(1), dim3(64), 0, st, c, dx, dy);
     return hipGetLastError();
 }
+
+#endif  // CFD_FAST_TU
 
 }  // namespace cfd
