@@ -9,7 +9,7 @@
  * advances all live grids together in chunks of whole time steps, writes a frame of each grid
  * every frame_every of its steps to pinned host memory while stepping continues, and retires
  * grids as they finish. Each request's final state is bitwise the state the plain single-grid
- * solver (cfdart.h) produces for the same request alone; cfdart_serve_one_at_a_time runs that
+ * solver (cfdart.h) produces for the same request alone, in the same arithmetic (opts.arith); cfdart_serve_one_at_a_time runs that
  * plain solver request by request for comparison.
  *
  * Conventions as in cfdart.h: 0 or a negative CFDART_E_* code.
