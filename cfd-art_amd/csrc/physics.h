@@ -171,7 +171,8 @@ struct MixEos {
 
 // Conserved -> primitive (E/headers/misc_euler.hpp:63-78, A/headers/misc.hpp:130-147).
 // Also returns the sound speed, which the Jacobian of the same state needs
-// (E/headers/misc_euler.hpp:110, A/headers/misc.hpp:198 evaluate it from the same inputs).
+// (E/headers/misc_euler.hpp:110, A/headers/misc.hpp:198 evaluate it from the same inputs); the
+// fast arithmetic returns rho c^2 in its place, the Jacobian's only use of it, without the root.
 template <int M> struct Prim;
 template <> struct Prim<EULER> {
     double W[4];
@@ -184,7 +185,8 @@ template <> struct Prim<EULER> {
         double e = ddiv<F>(U[3], rho) - 0.5 * (u * u + v * v);
         double p = Eos<EULER>::pressure(m, rho, e);
         W[0] = rho; W[1] = u; W[2] = v; W[3] = p;
-        c = Eos<EULER>::soundspeed<F>(m, rho, p);
+        if constexpr (F) c = m.g1 * (p + m.pinf1);  // rho c^2
+        else c = Eos<EULER>::soundspeed<F>(m, rho, p);
     }
 };
 template <> struct Prim<ALLAIRE> {
@@ -199,7 +201,8 @@ template <> struct Prim<ALLAIRE> {
         Mix x = mixture<F>(m, U[5]);
         double p = MixEos::pressure<F>(x, rho, e);
         W[0] = U[0]; W[1] = U[1]; W[2] = u; W[3] = v; W[4] = p; W[5] = U[5];
-        c = MixEos::soundspeed<F>(x, rho, p);
+        if constexpr (F) c = x.k + p * (ddiv<F>(1.0, x.eps) + 1.0);  // rho c^2
+        else c = MixEos::soundspeed<F>(x, rho, p);
     }
 };
 
@@ -371,9 +374,24 @@ template <bool F = false> CFD_HD void side_from_cons_mix(const Mix& x, Side<ALLA
 }
 
 // Evolved state from its primitive vector T (step M4 result): p2c, then the M5 test; the
-// Allaire test snaps the state in place and the snapped state is what HLLC receives.
+// Allaire test snaps the state in place and the snapped state is what HLLC receives. The exact
+// arithmetic recovers HLLC's rho, u, p and c from the conserved state, as the reference does;
+// the fast arithmetic takes them from T (the recovery returns T up to its rounding), and the
+// test's e >= 0 becomes rho e >= 0 with rho > 0 (the recovery's e is NaN at rho = 0).
 template <bool F = false> CFD_HD void side_from_prim_(const Material& m, const double* T, Side<EULER>& s)
 {
+    if constexpr (F) {
+        double rho = T[0], u = T[1], v = T[2], p = T[3];
+        double rhoe = ddiv<F>(p + m.g1pinf1, m.g1m1);
+        s.U[0] = rho;
+        s.U[1] = rho * u;
+        s.U[2] = rho * v;
+        s.U[3] = rhoe + 0.5 * rho * (u * u + v * v);
+        s.rho = rho; s.u = u; s.p = p;
+        s.c = Eos<EULER>::soundspeed<F>(m, rho, p);
+        s.ok = rho > 0.0 && rhoe >= 0.0;
+        return;
+    }
     p2c<EULER, F>(m, T, s.U);
     side_from_cons<F>(m, s);
 }
@@ -383,14 +401,25 @@ template <bool F = false> CFD_HD void side_from_prim_(const Material& m, const d
     // changes z (then they are recomputed from the snapped z, as the reference does)
     double rho = T[0] + T[1];
     Mix x = mixture<F>(m, T[5]);
-    double e = MixEos::specific_ie<F>(x, rho, T[4]);
+    // rho e; the fast arithmetic without the division by rho that specific_ie's product undoes
+    double rhoe = F ? (T[4] + x.k) * x.eps : rho * MixEos::specific_ie<F>(x, rho, T[4]);
     s.U[0] = T[0];
     s.U[1] = T[1];
     s.U[2] = rho * T[2];
     s.U[3] = rho * T[3];
-    s.U[4] = rho * e + 0.5 * rho * (T[2] * T[2] + T[3] * T[3]);
+    s.U[4] = rhoe + 0.5 * rho * (T[2] * T[2] + T[3] * T[3]);
     s.U[5] = T[5];
     snap(s.U);
+    if constexpr (F) {
+        if (s.U[0] == T[0] && s.U[1] == T[1] && s.U[5] == T[5]) {  // snapping changed nothing
+            double p = T[4];
+            s.rho = rho; s.u = T[2]; s.p = p;
+            s.c = MixEos::soundspeed<F>(x, rho, p);
+            s.eps = x.eps; s.k = x.k;
+            s.ok = T[0] >= 0.0 && T[1] >= 0.0 && rho > 0.0 && rhoe >= 0.0 && T[5] >= 0.0 && T[5] <= 1.0;
+            return;
+        }
+    }
     if (s.U[5] != T[5]) x = mixture<F>(m, s.U[5]);
     side_from_cons_mix<F>(x, s);
 }
@@ -417,12 +446,22 @@ template <int M> struct CellStage {
 };
 
 // tmp = (s * A(W)) * del, summed over k = 0.. in order (Eigen's unrolled lazy product of a
-// fixed-size matrix and a vector: E/headers/flux_solver_MUSCLhancock.hpp:197,201).
+// fixed-size matrix and a vector: E/headers/flux_solver_MUSCLhancock.hpp:197,201), zero entries
+// included. The fast arithmetic leaves out the zero entries' products (each adds a signed zero
+// to a finite sum) and takes rho c^2 from the primitive state's c field.
 template <bool F = false> CFD_HD void jac_times(const double* W, double c, double s, const double* d, double* tmp, Model<EULER>*)
 {
     double rho = W[0], u = W[1];
     double a00 = s * u, a01 = s * rho, z0 = s * 0.0;
     double a13 = s * ddiv<F>(1.0, rho);
+    if constexpr (F) {
+        double a31 = s * c;
+        tmp[0] = a01 * d[1] + a00 * d[0];
+        tmp[1] = a13 * d[3] + a00 * d[1];
+        tmp[2] = a00 * d[2];
+        tmp[3] = a00 * d[3] + a31 * d[1];
+        return;
+    }
     double a31 = s * (rho * c * c);
     tmp[0] = z0 * d[3] + (z0 * d[2] + (a01 * d[1] + a00 * d[0]));
     tmp[1] = a13 * d[3] + (z0 * d[2] + (a00 * d[1] + z0 * d[0]));
@@ -435,6 +474,16 @@ template <bool F = false> CFD_HD void jac_times(const double* W, double c, doubl
     double su = s * W[2], z0 = s * 0.0;
     double a02 = s * W[0], a12 = s * W[1];
     double a24 = s * ddiv<F>(1.0, rho);
+    if constexpr (F) {
+        double a42 = s * c;
+        tmp[0] = a02 * d[2] + su * d[0];
+        tmp[1] = a12 * d[2] + su * d[1];
+        tmp[2] = a24 * d[4] + su * d[2];
+        tmp[3] = su * d[3];
+        tmp[4] = su * d[4] + a42 * d[2];
+        tmp[5] = su * d[5];
+        return;
+    }
     double a42 = s * (rho * c * c);
     tmp[0] = z0 * d[5] + (z0 * d[4] + (z0 * d[3] + (a02 * d[2] + (z0 * d[1] + su * d[0]))));
     tmp[1] = z0 * d[5] + (z0 * d[4] + (z0 * d[3] + (a12 * d[2] + (su * d[1] + z0 * d[0]))));
@@ -536,6 +585,36 @@ CFD_HD void hllc(const Material& m, int wave_speeds, const Side<M>& L, const Sid
     }
     double us = ddiv<F>(pR - pL + rhoL * uL * (SL - uL) - rhoR * uR * (SR - uR),
                      rhoL * (SL - uL) - rhoR * (SR - uR));
+    if constexpr (F) {
+        // The weights (1 + sg)/2 and (1 - sg)/2 below are exactly 1 and 0: the fast arithmetic
+        // forms the upwind side's star state and flux only (the other term adds a signed zero).
+        const bool left = !signbit(us);
+        const Side<M>& W = left ? L : R;
+        const double S = left ? SL : SR, rho = left ? rhoL : rhoR, u = left ? uL : uR, p = left ? pL : pR;
+        const double mS = left ? smin(0.0, SL) : smax(0.0, SR);
+        const double f = ddiv<F>(S - u, S - us);
+        const double e = W.U[NC == 4 ? 3 : 4] + (us - u) * (rho * us + ddiv<F>(p, S - u));
+        for (int k = 0; k < NC; ++k) {
+            double st, fl;
+            if (M == ALLAIRE && k == 5) {
+                o.F[k] = 0.0;
+                continue;
+            } else if (k == MN) {
+                st = f * rho * us;
+                fl = W.U[k] * u + p;
+            } else if (k == NC - 1 - (M == ALLAIRE ? 1 : 0)) {
+                st = f * e;
+                fl = (W.U[k] + p) * u;
+            } else {
+                st = f * W.U[k];
+                fl = W.U[k] * u;
+            }
+            o.F[k] = fl + mS * (st - W.U[k]);
+        }
+        o.ustar = us;
+        o.zstar = M == ALLAIRE ? W.U[NC - 1] : 0.0;
+        return;
+    }
     double fL = ddiv<F>(SL - uL, SL - us);
     double fR = ddiv<F>(SR - uR, SR - us);
     double sg = copysign(1.0, us);
