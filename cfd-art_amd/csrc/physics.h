@@ -494,8 +494,11 @@ template <bool F = false> CFD_HD void jac_times(const double* W, double c, doubl
 }
 
 // M3 test of one extrapolated state (Allaire: the snapping overload on a temporary).
+// The fast arithmetic decides the Euler test on T itself: for rho > 0, e of p2c(T) has the sign
+// of p + gamma pinf (at rho = 0 the recovery's e is NaN, so the state fails).
 template <bool F = false> CFD_HD bool extrap_physical(const Material& m, const double* T, Model<EULER>*)
 {
+    if constexpr (F) return T[0] > 0.0 && T[3] + m.g1pinf1 >= 0.0;
     if (prim_clearly_physical(T[0], T[1], T[2], T[3], m.g1pinf1, m.g1m1)) return true;
     double U[4];
     p2c<EULER, F>(m, T, U);
@@ -683,7 +686,12 @@ CFD_HD int face_stage(const Material& m, const SchemeOpts& o, double s,
 // Euler repair (E/sourcecode/euler_rowsweep.cpp:47-82): 'old' and 'fut' in the unrotated frame.
 template <bool F = false> CFD_HD bool euler_repair(const Material& m, const double* old, double* fut)
 {
-    if (cons_clearly_physical(fut[0], fut[1], fut[2], fut[3])) return false;
+    if constexpr (F) {
+        // e >= 0 with rho > 0 is E rho - (mu^2 + mv^2)/2 >= 0; other states take the full test
+        if (fut[0] > 0.0 && fut[3] * fut[0] - 0.5 * (fut[1] * fut[1] + fut[2] * fut[2]) >= 0.0) return false;
+    } else if (cons_clearly_physical(fut[0], fut[1], fut[2], fut[3])) {
+        return false;
+    }
     double rho = fut[0];
     double u = ddiv<F>(fut[1], rho);
     double v = ddiv<F>(fut[2], rho);
