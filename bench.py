@@ -8,7 +8,8 @@ uploaded before the timed region, dt never leaves the device.
 
 Single-grid workloads are timed in both arithmetic modes of the library: the exact one (every
 division and square root correctly rounded, bitwise the reference's arithmetic) and the fast one
-(divisions and square roots within a few ulp). The line reports the fast mode, with its
+(divisions and square roots within a few ulp, fused multiply-adds, no work that only reproduces
+the exact arithmetic's roundings). The line reports the fast mode, with its
 per-field relative L1 and Linf against the exact mode's state after the same steps (the accuracy
 bar is 1e-10) and the exact mode's own figures beside it; --arith exact reports the exact mode
 alone.

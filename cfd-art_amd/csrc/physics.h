@@ -3,8 +3,9 @@
 //
 // Every expression keeps the reference's left-to-right C++ evaluation order so that, built
 // with -ffp-contract=off, the results of the exact arithmetic (template argument F = false, the
-// default) are bitwise those of the reference arithmetic; the fast arithmetic (F = true) differs
-// only in ddiv and dsqrt:
+// default) are bitwise those of the reference arithmetic. The fast arithmetic (F = true) differs
+// in ddiv and dsqrt and, in its `if constexpr (F)` branches, skips work that only reproduces the
+// exact arithmetic's roundings (each such shortcut changes values by rounding only):
 //   EOS            E/headers/stiffened_gas_eos.hpp:57-70, A/headers/mixturemodel.hpp:18-52
 //   state algebra  E/headers/misc_euler.hpp:20-131, A/headers/misc.hpp:31-266
 //   limiter        E/headers/flux_solver_MUSCLhancock.hpp:67-89 (+ van Leer, not in the reference)
