@@ -3,9 +3,8 @@ the numerator times the divisor's reciprocal refined by one Newton step, square 
 Goldschmidt step, products may fuse into sums, and work that only reproduces the exact
 arithmetic's roundings is skipped. The results are not bitwise the reference's arithmetic; the
 bar is per-field relative L1 and Linf within TOL = 1e-10 of the oracle on the same seeded
-inputs: euler_2048 and
-allaire_1024 after 100 alternating steps, the first-order Godunov scheme, every tile shape on a
-ragged grid and every splitting. Mass and energy stay conserved to round-off in a closed box, a
+inputs: euler_2048 and allaire_1024 after 100 alternating steps, the first-order Godunov scheme,
+every tile shape on a ragged grid and every splitting. Mass and energy stay conserved to round-off in a closed box, a
 free stream is preserved bitwise, and the exact mode on the same handle is bitwise again. A batch
 and the serving loop in the fast arithmetic give bitwise the lone fast runs."""
 import math
