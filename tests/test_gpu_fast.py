@@ -140,6 +140,45 @@ def test_free_stream(gpu):
     assert bitwise_equal(interior(g.download()), interior(h))
 
 
+def _riemann_x(nx, ny, left, right):
+    """Euler states (rho, u, p) left and right of x = 1/2, transmissive boundaries."""
+    d = cfdart.make_desc(cfdart.EULER, nx, ny, lx=1.0, ly=ny / nx, gamma1=1.4, bc=(0, 0, 0, 0),
+                         limiter=cfdart.VANLEER)
+    h = np.zeros((ny + 4, nx + 4, 4))
+    for j in range(nx):
+        rho, u, p = left if (j + 0.5) / nx < 0.5 else right
+        h[2:-2, 2 + j] = (rho, rho * u, 0.0, p / 0.4 + 0.5 * rho * u * u)
+    return d, h
+
+
+def test_evolve_fallbacks(gpu):
+    """A near-vacuum double rarefaction (rho 1e-3, u -20, p 1e-6 against rho 1, u 20, p 1): the
+    evolved half-step states leave the physical set and the faces fall back to the cells' own
+    states (the oracle counts 368 such faces in 60 steps). The exact arithmetic gives the
+    oracle's bits and counters; the fast one takes its positivity decisions on the half-step
+    states' primitives and stays within TOL."""
+    d, h = _riemann_x(128, 8, (1e-3, -20.0, 1e-6), (1.0, 20.0, 1.0))
+    ora = oracle.Sim(d, h)
+    ora.run(cfdart.ALTERNATING, 60, cfl=0.8)
+    assert ora.counters()[1] > 0
+    for mode in (cfdart.ARITH_EXACT, FAST):
+        print(f"case evolve fallbacks, arith={mode}", flush=True)
+        g = cfdart.Grid(d)
+        g.set_arith(mode)
+        g.upload(h)
+        t, n = g.run(cfdart.ALTERNATING, 60, cfl=0.8)
+        print(f"counters gpu {g.stats()} oracle {ora.counters()}", flush=True)
+        if mode == FAST:
+            assert n == 60 and abs(t - ora.t) <= 1e-12 * ora.t
+            _within(g.download(), ora.grid, "evolve fallbacks fast")
+            assert g.stats()[1] > 0
+        else:
+            assert n == 60 and t == ora.t
+            assert bitwise_equal(interior(g.download()), interior(ora.grid))
+            assert g.stats() == ora.counters()
+        g.close()
+
+
 def test_exact_again_on_the_same_handle(gpu):
     d, h = state(0, 130, 70, cfdart.MUSCL_HANCOCK, seed=5)
     print("case fast then exact arithmetic on one handle", flush=True)
