@@ -298,11 +298,19 @@ CFD_HD bool cons_clearly_physical(double rho, double mu, double mv, double E)
 }
 
 // Allaire post-update test (after snapping): a division-free screen, then the exact test.
+// The fast arithmetic decides e >= 0 as E rho - (mu^2 + mv^2)/2 >= 0 with rho > 0.
 template <bool F = false> CFD_HD bool allaire_physical_screened(const double* U)
 {
-    if (U[0] >= 0.0 && U[1] >= 0.0 && U[5] >= 0.0 && U[5] <= 1.0 &&
-        cons_clearly_physical(U[0] + U[1], U[2], U[3], U[4]))
-        return true;
+    if constexpr (F) {
+        const double rho = U[0] + U[1];
+        if (U[0] >= 0.0 && U[1] >= 0.0 && U[5] >= 0.0 && U[5] <= 1.0 && rho > 0.0 &&
+            U[4] * rho - 0.5 * (U[2] * U[2] + U[3] * U[3]) >= 0.0)
+            return true;
+    } else {
+        if (U[0] >= 0.0 && U[1] >= 0.0 && U[5] >= 0.0 && U[5] <= 1.0 &&
+            cons_clearly_physical(U[0] + U[1], U[2], U[3], U[4]))
+            return true;
+    }
     return is_physical<ALLAIRE, F>(U);
 }
 
@@ -507,10 +515,17 @@ template <bool F = false> CFD_HD bool extrap_physical(const Material& m, const d
 }
 template <bool F = false> CFD_HD bool extrap_physical(const Material& m, const double* T, Model<ALLAIRE>*)
 {
-    // no snapping can occur when both partial densities and z are away from 0 (and z from 1)
-    if (T[0] >= 1e-12 && T[1] >= 1e-12 && T[5] >= 1e-12 && T[5] <= 1.0 - 1e-12 &&
-        prim_clearly_physical(T[0] + T[1], T[2], T[3], T[4], m.kmin, ddiv<F>(1.0, m.epsmin)))
-        return true;
+    // no snapping can occur when both partial densities and z are away from 0 (and z from 1);
+    // then the fast arithmetic decides on T itself: e of p2c(T) has the sign of p + k(z), and the
+    // mixture k(z) is a weighted mean of gamma_i pinf_i, so p + kmin >= 0 is sufficient
+    if constexpr (F) {
+        if (T[0] >= 1e-12 && T[1] >= 1e-12 && T[5] >= 1e-12 && T[5] <= 1.0 - 1e-12 && T[4] + m.kmin >= 0.0)
+            return true;
+    } else {
+        if (T[0] >= 1e-12 && T[1] >= 1e-12 && T[5] >= 1e-12 && T[5] <= 1.0 - 1e-12 &&
+            prim_clearly_physical(T[0] + T[1], T[2], T[3], T[4], m.kmin, ddiv<F>(1.0, m.epsmin)))
+            return true;
+    }
     double U[6];
     p2c<ALLAIRE, F>(m, T, U);
     snap(U);
