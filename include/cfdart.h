@@ -143,7 +143,7 @@ int cfdart_set_tile(cfdart_grid* g, int y_strip_rows, int x_rows_per_wave);
  * primitive round trips of the half-step states, products with zero Jacobian entries); 110 steps
  * of the 2048^2 Euler and 1024^2 Allaire workloads stay within 5.6e-14 relative L1 and 8.2e-13
  * relative Linf of the exact results per field (the accuracy bar is 1e-10), and their sweeps
- * take 22-28 % less time. The
+ * take 19-30 % less time. The
  * per-cell wave speeds of the time step follow the mode (so a run's result does not depend on how
  * it is split into cfdart_run calls); the fold of the maxima into dt is exact. Batches
  * (cfdart_batch_set_arith) and the serving loop (cfdart_serve_opts.arith) take the mode too.
