@@ -6,7 +6,7 @@
 // default) are bitwise those of the reference arithmetic. The fast arithmetic (F = true) differs
 // in ddiv and dsqrt and, in its `if constexpr (F)` branches, skips work that only reproduces the
 // exact arithmetic's roundings (each such shortcut changes values by rounding only, and a
-// positivity decision only where its quantity is within rounding of the threshold):
+// positivity decision only where its quantity is within rounding of the threshold). Reference:
 //   EOS            E/headers/stiffened_gas_eos.hpp:57-70, A/headers/mixturemodel.hpp:18-52
 //   state algebra  E/headers/misc_euler.hpp:20-131, A/headers/misc.hpp:31-266
 //   limiter        E/headers/flux_solver_MUSCLhancock.hpp:67-89 (+ van Leer, not in the reference)
