@@ -118,17 +118,18 @@ def bench_batched(args):
     # Tile shapes for the batch: tuned per model on a batch of that model's grids alone (the
     # per-grid table above is tuned for a grid that has the GPU to itself; in a batch the
     # per-tile cost of the work-list favours larger tiles).
+    mode = cfdart.ARITH_FAST if args.arith == "fast" else cfdart.ARITH_EXACT
     batch_shape = {}
     for model in (cfdart.EULER, cfdart.ALLAIRE):
         sub = [c for c in cases if c[0].model == model]
         if not sub:
             continue
         tb = cfdart.Batch(len(sub))
+        tb.set_arith(mode)  # tuned in the arithmetic the batch runs
         ts = [tb.add(d, h, splitting=cfdart.ALTERNATING, max_steps=steps, cfl=cfl) for d, h, cfl in sub]
         batch_shape.update(cfdart.autotune.tune_batch(tb, {model: ts}, steps=3, log=log))
         tb.close()
 
-    mode = cfdart.ARITH_FAST if args.arith == "fast" else cfdart.ARITH_EXACT
     b = cfdart.Batch(len(cases))
     b.set_arith(mode)
     slots = []
