@@ -108,7 +108,8 @@ def bench_batched(args):
     import cfdart
 
     cases = cfdart.configs.batched_mix(args.grids)
-    table = cfdart.autotune.tune(classes=(128, 256, 512, 1024), log=log)
+    mode = cfdart.ARITH_FAST if args.arith == "fast" else cfdart.ARITH_EXACT
+    table = cfdart.autotune.tune(classes=(128, 256, 512, 1024), log=log, arith=mode)
     steps = args.batch_steps
     cells = [c[0].nx * c[0].ny for c in cases]
     work = float(sum(cells)) * steps
@@ -118,7 +119,6 @@ def bench_batched(args):
     # Tile shapes for the batch: tuned per model on a batch of that model's grids alone (the
     # per-grid table above is tuned for a grid that has the GPU to itself; in a batch the
     # per-tile cost of the work-list favours larger tiles).
-    mode = cfdart.ARITH_FAST if args.arith == "fast" else cfdart.ARITH_EXACT
     batch_shape = {}
     for model in (cfdart.EULER, cfdart.ALLAIRE):
         sub = [c for c in cases if c[0].model == model]
@@ -326,16 +326,24 @@ def main():
     d, h = single["cfg"]()
     cfl = single["cfl"]
     cells = d.nx * d.ny
-    if args.tile:
-        yh, xr = (int(v) for v in args.tile.split(","))
-        tuned = {"y_strip_rows": yh, "x_rows_per_wave": xr}
-    else:
-        tuned = cfdart.autotune.tune_one(single["model"], d.nx, log=log, case=(d, h))
+    modes = ["exact", "fast"] if args.arith == "fast" else ["exact"]
+    arith_of = {"exact": cfdart.ARITH_EXACT, "fast": cfdart.ARITH_FAST}
+    # each arithmetic runs at the tile shape tuned in that arithmetic
+    tiles = {}
+    for m in modes:
+        if args.tile:
+            yh, xr = (int(v) for v in args.tile.split(","))
+            tiles[m] = {"y_strip_rows": yh, "x_rows_per_wave": xr}
+        else:
+            tiles[m] = cfdart.autotune.tune_one(single["model"], d.nx, log=log, case=(d, h), arith=arith_of[m])
     bytes_per_launch = 16.0 * single["nc"] * cells  # one FP64 read + write of the state per cell
 
-    def timed(arith):
+    def timed(mode):
         """The workload's timed run in one arithmetic mode: warm-up steps, clocks brought up on a
-        scratch copy, then the timed steps with HIP events around every sweep."""
+        scratch copy, the per-kernel event averages from a pass on the scratch copy, then the
+        timed steps without events."""
+        arith = arith_of[mode]
+        tuned = tiles[mode]
         g = cfdart.Grid(d)
         g.set_tile(tuned["y_strip_rows"], tuned["x_rows_per_wave"])
         g.set_arith(arith)
@@ -360,15 +368,18 @@ def main():
             log(f"{args.workload}: the run to t = {T_end} takes {n_s} steps, all timed")
         else:
             scratch.run(cfdart.ALTERNATING, 100, cfl=cfl)
+        # the timed steps run without events; the per-kernel averages come from a second pass of
+        # the same length on the scratch grid, with an event pair around every sweep (the events
+        # add a few microseconds per step, which the timed steps must not carry)
+        scratch.set_profiling(True)
+        scratch.run(cfdart.ALTERNATING, steps, T=math.inf, cfl=cfl, sync=False)
+        prof = scratch.profile()
         scratch.close()
-        g.set_profiling(True)
         g.sync()
         t0 = time.perf_counter()
         g.run(cfdart.ALTERNATING, steps, T=T_end, cfl=cfl, sync=False)
         g.sync()
         el = time.perf_counter() - t0
-        prof = g.profile()
-        g.set_profiling(False)
         t_sim, n_sim = g.run(cfdart.ALTERNATING, 0, T=T_end, cfl=cfl)
         per_kernel = {}
         for axis, (ms, nl) in prof.items():
@@ -383,14 +394,14 @@ def main():
 
     if not math.isfinite(T_end):
         log(f"{args.workload}: warm-up {args.warmup} steps, timed {args.steps} steps")
-    modes = ["exact", "fast"] if args.arith == "fast" else ["exact"]
-    res = {m: timed(cfdart.ARITH_FAST if m == "fast" else cfdart.ARITH_EXACT) for m in modes}
+    res = {m: timed(m) for m in modes}
     for m in modes:
         log(f"{args.workload}: {m} arithmetic {res[m]['value'] / 1e9:.2f} G cell-updates/s, "
             f"x {res[m]['kernels']['x']['avg_us']:.1f} us, y {res[m]['kernels']['y']['avg_us']:.1f} us")
     head = res[modes[-1]]
     per_kernel = head["kernels"]
     dom = max(per_kernel, key=lambda a: per_kernel[a]["avg_us"] * per_kernel[a]["launches"])
+    tuned = tiles[modes[-1]]
     transposed = (dom == "y" and tuned["y_strip_rows"] < 0 and not args.strip_rows
                   and d.flux_solver == cfdart.MUSCL_HANCOCK)  # the Godunov scheme runs y strips
     kname = f"sweep_{'yt' if transposed else dom}_kernel<{single['tag']}>"
@@ -424,6 +435,8 @@ def main():
         ex = res["exact"]
         ex_dom = ex["kernels"][dom]["gbs"]
         out["exact_arith"] = {"value": ex["value"], "ms_per_step": 1e3 * ex["el"] / ex["steps"],
+                              "tile": {"y_strip_rows": tiles["exact"]["y_strip_rows"],
+                                       "x_rows_per_wave": tiles["exact"]["x_rows_per_wave"]},
                               "kernels": ex["kernels"], "roofline_frac": ex_dom / HBM_PEAK_GBS,
                               "note": "every division and square root correctly rounded: bitwise the "
                                       "reference's arithmetic (the oracle), the library's default"}
@@ -441,8 +454,14 @@ def main():
             "rel_l1_max": max(e[0] for e in errs) if errs else None,
             "rel_linf_max": max(e[1] for e in errs) if errs else None,
             "t_end_exact": ex["t_sim"]}
-        log(f"{args.workload}: fast vs exact after {head['n_sim']} steps: rel L1 max "
-            f"{out['accuracy']['rel_l1_max']:.2e}, rel Linf max {out['accuracy']['rel_linf_max']:.2e}")
+        if errs:
+            log(f"{args.workload}: fast vs exact after {head['n_sim']} steps: rel L1 max "
+                f"{out['accuracy']['rel_l1_max']:.2e}, rel Linf max {out['accuracy']['rel_linf_max']:.2e}")
+        else:
+            # a run to T whose dt sequences differ by rounding can end on different step counts
+            out["accuracy"]["step_mismatch"] = {"fast": head["n_sim"], "exact": ex["n_sim"]}
+            log(f"{args.workload}: fast and exact runs ended on different step counts "
+                f"({head['n_sim']} vs {ex['n_sim']}); no field comparison")
     if args.cpu_steps > 0 and single.get("cpu"):
         c = single["cpu"]
         out["cpu_baseline"] = cpu_baseline(single["cfg"], cfl, c["steps"], c["steps_1t"], T=single.get("T", math.inf),

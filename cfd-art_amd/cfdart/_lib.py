@@ -125,7 +125,7 @@ def lib():
         L.cfdart_host_alloc.restype = C.c_void_p
         L.cfdart_host_free.argtypes = [C.c_void_p]
         L.cfdart_host_free.restype = None
-        L.cfdart_time_tile.argtypes = [P(Desc), dp, C.c_int, C.c_int, C.c_int, dp, dp]
+        L.cfdart_time_tile.argtypes = [P(Desc), dp, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
         _lib = L
     return _lib
 
@@ -419,11 +419,11 @@ class Batch:
         _check(lib().cfdart_batch_frames_done(self.h))
 
 
-def time_tile(desc, y_strip_rows, x_rows_per_wave, reps=5, state=None):
+def time_tile(desc, y_strip_rows, x_rows_per_wave, reps=5, state=None, arith=ARITH_EXACT):
     ux = C.c_double()
     uy = C.c_double()
     sp = None if state is None else _dptr(np.ascontiguousarray(state, dtype=np.float64))
-    _check(lib().cfdart_time_tile(C.byref(desc), sp, y_strip_rows, x_rows_per_wave, reps, C.byref(ux),
+    _check(lib().cfdart_time_tile(C.byref(desc), sp, y_strip_rows, x_rows_per_wave, reps, arith, C.byref(ux),
                                   C.byref(uy)))
     return ux.value, uy.value
 

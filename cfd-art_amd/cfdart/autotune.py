@@ -37,37 +37,37 @@ def _around(best, lo, step):
                    for f in (0.5, 0.625, 0.75, 0.875, 1.0, 1.125, 1.25, 1.5, 1.75, 2.0)})
 
 
-def tune_one(model, n, reps=5, log=None, case=None):
+def tune_one(model, n, reps=5, log=None, case=None, arith=L.ARITH_EXACT):
     """Coarse search over powers of two, then a finer search around the best of each axis: the
     number of wavefronts a shape launches against the chip's resident-wave slots (256 CUs x
     4 SIMDs x waves per SIMD) decides how full the last round of waves is, which powers of two
-    alone do not resolve."""
+    alone do not resolve. The shapes are timed in the arithmetic the grid will run."""
     d, h = case if case is not None else _case(model, n)
-    tx = {xr: L.time_tile(d, 32, xr, reps, h)[0] for xr in X_ROWS}
-    ty = {yh: L.time_tile(d, yh, 4, reps, h)[1] for yh in Y_STRIPS}
+    tx = {xr: L.time_tile(d, 32, xr, reps, h, arith)[0] for xr in X_ROWS}
+    ty = {yh: L.time_tile(d, yh, 4, reps, h, arith)[1] for yh in Y_STRIPS}
     for xr in _around(min(tx, key=tx.get), 1, 1):
         if xr not in tx:
-            tx[xr] = L.time_tile(d, 32, xr, reps, h)[0]
+            tx[xr] = L.time_tile(d, 32, xr, reps, h, arith)[0]
     for yh in _around(min(ty, key=ty.get), 4, 2):
         if yh not in ty:
-            ty[yh] = L.time_tile(d, yh, 4, reps, h)[1]
+            ty[yh] = L.time_tile(d, yh, 4, reps, h, arith)[1]
     for yt in (L.Y_TRANSPOSED, L.Y_TRANSPOSED_NARROW):
-        ty[yt] = L.time_tile(d, yt, 4, reps, h)[1]
+        ty[yt] = L.time_tile(d, yt, 4, reps, h, arith)[1]
     bx = min(tx, key=tx.get)
     by = min(ty, key=ty.get)
     if log:
-        log(f"autotune model={model} n={n}: x {{{', '.join(f'{k}:{v:.1f}us' for k, v in sorted(tx.items()))}}} -> {bx}; "
+        log(f"autotune model={model} n={n} arith={arith}: x {{{', '.join(f'{k}:{v:.1f}us' for k, v in sorted(tx.items()))}}} -> {bx}; "
             f"y {{{', '.join(f'{k}:{v:.1f}us' for k, v in sorted(ty.items()))}}} -> {by}")
     return {"y_strip_rows": by, "x_rows_per_wave": bx, "x_us": tx[bx], "y_us": ty[by]}
 
 
-def tune(models=(L.EULER, L.ALLAIRE), classes=SIZE_CLASSES, reps=5, cache=None, log=None):
+def tune(models=(L.EULER, L.ALLAIRE), classes=SIZE_CLASSES, reps=5, cache=None, log=None, arith=L.ARITH_EXACT):
     """Returns {model: {size_class: {...}}}; reads/writes the JSON cache when given."""
     if cache and os.path.exists(cache):
         with open(cache) as f:
             table = json.load(f)
         return {int(m): {int(c): v for c, v in t.items()} for m, t in table.items()}
-    table = {m: {c: tune_one(m, c, reps, log) for c in classes} for m in models}
+    table = {m: {c: tune_one(m, c, reps, log, arith=arith) for c in classes} for m in models}
     if cache:
         os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)
         with open(cache, "w") as f:

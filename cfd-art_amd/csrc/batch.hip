@@ -723,8 +723,8 @@ void cfdart_host_free(void* p)
     if (p) (void)hipHostFree(p);
 }
 
-int cfdart_time_tile(const cfdart_desc* d, const double* host_state, int yH, int xrows, int reps, double* us_x,
-                     double* us_y)
+int cfdart_time_tile(const cfdart_desc* d, const double* host_state, int yH, int xrows, int reps, int arith,
+                     double* us_x, double* us_y)
 {
     if (!d || !us_x || !us_y || reps < 1) return bfail(CFDART_E_ARG, "bad argument");
     cfdart_grid* g = nullptr;
@@ -746,6 +746,7 @@ int cfdart_time_tile(const cfdart_desc* d, const double* host_state, int yH, int
         for (int k = 0; k < nc; ++k) h[c * nc + k] = U[k];
     rc = cfdart_upload(g, host_state ? host_state : h.data());
     if (!rc) rc = cfdart_set_tile(g, yH, xrows);
+    if (!rc) rc = cfdart_set_arith(g, arith);
     if (!rc) rc = cfdart_set_profiling(g, 1);
     double dt = 0.0;
     if (!rc) rc = cfdart_compute_dt(g, 10, 0.0, INFINITY, 0.5, &dt, nullptr);

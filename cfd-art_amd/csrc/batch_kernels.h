@@ -75,7 +75,7 @@ batch_x_kernel(BatchGrid* grids, const Tile* tiles, const BatchQueue* q, int pha
     const GridView g = batch_view(bg, phase);
     const int xrows = bg->xrows;
     Tally ta;
-    x_tile<M, LAST, F>(g, bg->ctrl.dt, tl.a, (tl.b * 4 + (threadIdx.x >> 6)) * xrows, xrows, ta);
+    x_tile<M, LAST, F>(g, bg->ctrl.dt, tl.a, (tl.b * 4 + wave_index()) * xrows, xrows, ta);
     ta.flush(&bg->ctrl, LAST, t);
 }
 

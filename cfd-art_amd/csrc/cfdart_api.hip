@@ -75,6 +75,7 @@ int check_desc(const cfdart_desc* d)
     if (d->model == CFDART_ALLAIRE && !(d->z_lo >= 0.0 && d->z_lo <= d->z_hi && d->z_hi <= 1.0))
         return fail(CFDART_E_ARG, "need 0 <= z_lo <= z_hi <= 1");
     if ((long long)d->nx * d->ny > (1LL << 31)) return fail(CFDART_E_ARG, "grid too large");
+    if (d->nx > (1 << 28)) return fail(CFDART_E_ARG, "nx must be at most 2^28 (a row's byte offsets are 32-bit)");
     return 0;
 }
 

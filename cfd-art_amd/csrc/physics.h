@@ -333,9 +333,24 @@ template <int M> struct Side {
     static constexpr int NC = Model<M>::NC;
     double U[NC];
     double rho, u, p, c;
+    double v;       // fast Euler evolved sides: the tangential velocity their U is formed from
     double eps, k;  // Allaire: the mixture coefficients of z = U[5] (the PVRS speeds reuse them)
     bool ok;
 };
+
+// The conserved state of a fast-arithmetic Euler evolved side from its rho, u, v, p; returns
+// rho e = (p + gamma pinf)/(gamma - 1), without the division by rho that specific_ie's product
+// undoes. A side sent to a neighbouring lane is rebuilt there by the same function.
+CFD_HD double euler_fast_cons(const Material& m, Side<EULER>& s)
+{
+    const double rho = s.rho, u = s.u, v = s.v;
+    const double rhoe = ddiv<true>(s.p + m.g1pinf1, m.g1m1);
+    s.U[0] = rho;
+    s.U[1] = rho * u;
+    s.U[2] = rho * v;
+    s.U[3] = rhoe + 0.5 * rho * (u * u + v * v);
+    return rhoe;
+}
 
 // From a conserved state: HLLC's own primitive recovery (identical to the physical-state
 // test's divisions, which the compiler shares).
@@ -391,15 +406,10 @@ template <bool F = false> CFD_HD void side_from_cons_mix(const Mix& x, Side<ALLA
 template <bool F = false> CFD_HD void side_from_prim_(const Material& m, const double* T, Side<EULER>& s)
 {
     if constexpr (F) {
-        double rho = T[0], u = T[1], v = T[2], p = T[3];
-        double rhoe = ddiv<F>(p + m.g1pinf1, m.g1m1);
-        s.U[0] = rho;
-        s.U[1] = rho * u;
-        s.U[2] = rho * v;
-        s.U[3] = rhoe + 0.5 * rho * (u * u + v * v);
-        s.rho = rho; s.u = u; s.p = p;
-        s.c = Eos<EULER>::soundspeed<F>(m, rho, p);
-        s.ok = rho > 0.0 && rhoe >= 0.0;
+        s.rho = T[0]; s.u = T[1]; s.v = T[2]; s.p = T[3];
+        const double rhoe = euler_fast_cons(m, s);
+        s.c = Eos<EULER>::soundspeed<F>(m, s.rho, s.p);
+        s.ok = s.rho > 0.0 && rhoe >= 0.0;
         return;
     }
     p2c<EULER, F>(m, T, s.U);

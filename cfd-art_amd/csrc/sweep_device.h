@@ -10,52 +10,92 @@ namespace cfd {
 
 // ------------------------------------------------------------------ lane exchange
 
-__device__ __forceinline__ double lane_from_left(double x)  // lane i receives lane i-1
+// Wavefront rotations by one lane (DPP wave_ror:1 and wave_rol:1): lane i receives lane i-1
+// (from_left) or lane i+1 (from_right), and lanes 0 and 63 receive lanes 63 and 0. Every lane
+// has a source lane, so each 32-bit half is one v_mov_b32_dpp that writes its destination
+// outright (a shift leaves a lane without a source unwritten, which costs a copy of a fallback
+// value into the destination first). Every lane of the wavefront must be active. What lanes 0
+// and 63 receive is no neighbour's: the line computations below never store those lanes' cells
+// and never use a value derived from them in a stored cell.
+__device__ __forceinline__ int rot_from_left_i(int x) { return __builtin_amdgcn_mov_dpp(x, 0x13C, 0xf, 0xf, false); }
+__device__ __forceinline__ int rot_from_right_i(int x) { return __builtin_amdgcn_mov_dpp(x, 0x134, 0xf, 0xf, false); }
+__device__ __forceinline__ double rot_from_left(double x)
 {
-    int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x138, 0xf, 0xf, false);
-    int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x138, 0xf, 0xf, false);
-    return __hiloint2double(hi, lo);
+    return __hiloint2double(rot_from_left_i(__double2hiint(x)), rot_from_left_i(__double2loint(x)));
 }
-__device__ __forceinline__ double lane_from_right(double x)  // lane i receives lane i+1
+__device__ __forceinline__ double rot_from_right(double x)
 {
-    int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x130, 0xf, 0xf, false);
-    int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x130, 0xf, 0xf, false);
-    return __hiloint2double(hi, lo);
-}
-// As above, but a lane without a source lane (lane 0 shifting right, lane 63 shifting left)
-// keeps its own value instead of 0 (bound_ctrl off: the DPP move leaves such lanes unwritten).
-__device__ __forceinline__ double lane_from_left_or_self(double x)
-{
-    int lo = __builtin_amdgcn_update_dpp(__double2loint(x), __double2loint(x), 0x138, 0xf, 0xf, false);
-    int hi = __builtin_amdgcn_update_dpp(__double2hiint(x), __double2hiint(x), 0x138, 0xf, 0xf, false);
-    return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ double lane_from_right_or_self(double x)
-{
-    int lo = __builtin_amdgcn_update_dpp(__double2loint(x), __double2loint(x), 0x130, 0xf, 0xf, false);
-    int hi = __builtin_amdgcn_update_dpp(__double2hiint(x), __double2hiint(x), 0x130, 0xf, 0xf, false);
-    return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ int lane_from_left_or_self_i(int x)
-{
-    return __builtin_amdgcn_update_dpp(x, x, 0x138, 0xf, 0xf, false);
+    return __hiloint2double(rot_from_right_i(__double2hiint(x)), rot_from_right_i(__double2loint(x)));
 }
 
-// The right-hand side of the left neighbour's cell stage; lane 0 takes its own.
-template <int M> __device__ __forceinline__ void side_from_left(const Side<M>& s, Side<M>& o)
+// True on every lane when the predicate holds on any lane (a wavefront-uniform branch condition:
+// code under it runs with every lane active, so it may move values between lanes).
+__device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
+
+// The right-hand evolved side of the left neighbour's cell stage (its 'ok' flag travels
+// separately). Only what cannot be rebuilt is moved: rho is U[0] (Euler) or U[0] + U[1]
+// (Allaire) on every side, and a fast-arithmetic Euler side is a function of its rho, u, v, p.
+template <int M, bool F>
+__device__ __forceinline__ void side_from_left(const Material& m, const Side<M>& s, Side<M>& o)
 {
+    if constexpr (F && M == EULER) {
+        o.rho = rot_from_left(s.rho);
+        o.u = rot_from_left(s.u);
+        o.v = rot_from_left(s.v);
+        o.p = rot_from_left(s.p);
+        o.c = rot_from_left(s.c);
+        (void)euler_fast_cons(m, o);
+    } else {
 #pragma unroll
-    for (int k = 0; k < Model<M>::NC; ++k) o.U[k] = lane_from_left_or_self(s.U[k]);
-    o.rho = lane_from_left_or_self(s.rho);
-    o.u = lane_from_left_or_self(s.u);
-    o.p = lane_from_left_or_self(s.p);
-    o.c = lane_from_left_or_self(s.c);
-    if (M == ALLAIRE) {
-        o.eps = lane_from_left_or_self(s.eps);
-        o.k = lane_from_left_or_self(s.k);
+        for (int k = 0; k < Model<M>::NC; ++k) o.U[k] = rot_from_left(s.U[k]);
+        o.rho = M == EULER ? o.U[0] : o.U[0] + o.U[1];
+        o.u = rot_from_left(s.u);
+        o.p = rot_from_left(s.p);
+        o.c = rot_from_left(s.c);
+        if (M == ALLAIRE) {
+            o.eps = rot_from_left(s.eps);
+            o.k = rot_from_left(s.k);
+        }
     }
-    o.ok = lane_from_left_or_self_i(s.ok ? 1 : 0) != 0;
 }
+
+// The wavefront's index in its workgroup, as a wavefront-uniform value: rows derived from it
+// are uniform too, so a row's address is formed once in scalar registers.
+__device__ __forceinline__ int wave_index() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+
+// Row pointers in the global address space (the memory instructions then take a scalar base
+// address and a 32-bit per-lane offset).
+typedef __attribute__((address_space(1))) const double* grow_c;
+typedef __attribute__((address_space(1))) double* grow;
+
+// A wavefront-uniform row pointer kept in scalar registers (the loop optimiser would otherwise
+// carry per-lane 64-bit copies of each row pointer across the row loop).
+__device__ __forceinline__ unsigned long long uniform_addr(const void* p)
+{
+    const unsigned long long a = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+__device__ __forceinline__ grow_c uniform_row(const double* p) { return (grow_c)uniform_addr(p); }
+__device__ __forceinline__ grow uniform_row(double* p) { return (grow)uniform_addr(p); }
+
+// Element x (0 <= x < 2^28) of a uniform row: the lane adds a 32-bit byte offset.
+__device__ __forceinline__ grow_c row_elem(grow_c row, int x)
+{
+    return (grow_c)((__attribute__((address_space(1))) const char*)row + (unsigned)x * 8u);
+}
+__device__ __forceinline__ grow row_elem(grow row, int x)
+{
+    return (grow)((__attribute__((address_space(1))) char*)row + (unsigned)x * 8u);
+}
+
+// New states are written with ordinary stores, which leave them in the memory-side cache (256 MB
+// on MI355X) from which the next sweep reads them back. Nontemporal (streaming) stores bypass it:
+// each sweep then read its input from HBM, and the memory traffic of a 2048^2 Euler x sweep alone
+// took 56 instead of 44 us.
+__device__ __forceinline__ void store_state(double v, double* p) { *p = v; }
+__device__ __forceinline__ void store_state(double v, grow p) { *p = v; }
 
 // Workgroups go to the 8 XCDs (each with its own L2) round-robin in order of linear id.
 // xcd_order maps the linear id b of n workgroups to a tile order in which each XCD receives
@@ -148,6 +188,82 @@ __device__ __forceinline__ void update_cell(const GridView& g, double r, const d
     }
 }
 
+// ------------------------------------------------------------------ line step
+
+// One step of a line of 64 consecutive cells, one per lane, shared by the x sweep (a row) and
+// the transposed y sweep (a column in the rotated frame). U is the lane's conserved state in the
+// sweep's frame, with a reflective wall's ghost sign flip applied. Each lane forms its cell stage
+// (primitives, limited slopes from its neighbours' primitives, M3 flags, evolved sides) and the
+// face to its left from its left neighbour's right-hand side and its own left-hand side (M3 and M5
+// fallbacks, HLLC), receives the face to its right from the next lane, and updates its cell into
+// Un (bad: the Allaire physical-state test failed, tested where 'checked'). Returns the face's
+// fallback flags (bit 0 slope, bit 1 evolve). A stored cell's result needs its lane's neighbours
+// two lanes either side, so lanes 0, 1, 62 and 63 are halo lanes; lanes 0 and 63 receive values
+// from no neighbour and are never stored, and neither is anything derived from them.
+template <int M, bool F>
+__device__ __forceinline__ int line_step(const GridView& g, double s, double r, const double* U, bool checked,
+                                         double* Un, bool& bad)
+{
+    constexpr int NC = Model<M>::NC;
+    Prim<M> P;
+    P.template from_cons<F>(g.mat, U);
+    double Wm[NC], Wp[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) { Wm[k] = rot_from_left(P.W[k]); Wp[k] = rot_from_right(P.W[k]); }
+    CellStage<M> cs;
+    cell_stage<M, F>(g.mat, g.opt.limiter, s, P, Wm, Wp, cs);
+
+    // the face's left input: the left neighbour's right-hand side, with its M5 and M3 flags in
+    // one word; its right input: this cell's left-hand side
+    Side<M> L, R = cs.L;
+    side_from_left<M, F>(g.mat, cs.R, L);
+    const int fw = rot_from_left_i((cs.R.ok ? 1 : 0) | (cs.fR ? 2 : 0));
+    L.ok = (fw & 1) != 0;
+    int flags = 0;
+    // M3: both slopes of the face are zeroed when either extrapolation is unphysical. The left
+    // neighbour's primitives and sound speed are moved (again) only when some lane of the
+    // wavefront needs them, so they need not stay live across the cell stage.
+    const bool slope_fb = !((fw & 2) != 0 && cs.fL);
+    if (wave_any(slope_fb)) {
+        Prim<M> Pa;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) Pa.W[k] = rot_from_left(P.W[k]);
+        Pa.c = rot_from_left(P.c);
+        if (slope_fb) {
+            flags |= 1;
+            unsloped_side<M, F>(g.mat, s, Pa, true, L);
+            unsloped_side<M, F>(g.mat, s, P, false, R);
+        }
+    }
+    // M5: an unphysical evolved side sends the face to the raw stencil cells; the left
+    // neighbour's raw state is moved only when some lane needs it
+    const bool evolve_fb = !(L.ok && R.ok);
+    if (wave_any(evolve_fb)) {
+        double Ua[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) Ua[k] = rot_from_left(U[k]);
+        if (evolve_fb) {
+            flags |= 2;
+            side_from_raw<M, F>(g.mat, Ua, L);
+            side_from_raw<M, F>(g.mat, U, R);
+        }
+    }
+    FaceOut<M> fo;
+    hllc<M, F>(g.mat, g.opt.wave_speeds, L, R, fo);
+
+    // the face to the right, from the next lane (Allaire's sixth flux component is zero, and the
+    // z update replaces that component of the new state: it is not moved)
+    constexpr int NF = M == ALLAIRE ? 5 : NC;
+    double Fr[NC];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) Fr[k] = rot_from_right(fo.F[k]);
+    if (M == ALLAIRE) Fr[5] = 0.0;
+    const double usr = rot_from_right(fo.ustar);
+    const double zsr = M == ALLAIRE ? rot_from_right(fo.zstar) : 0.0;
+    update_cell<M, F>(g, r, U, fo.F, Fr, fo.ustar, usr, fo.zstar, zsr, Un, bad, checked);
+    return flags;
+}
+
 // ------------------------------------------------------------------ x sweep
 
 // One line of an x-sweep tile: the calling wavefront's lanes hold columns [tx*60-2, tx*60+62)
@@ -165,50 +281,19 @@ __device__ __forceinline__ void x_line(const GridView& g, double s, double r, in
     const bool face_counted = (lane >= 2 && lane < 62 && x <= g.nx) || (lane == 62 && x == g.nx);
     if (neg) U[MN] = -U[MN];
 
-    Prim<M> P;
-    P.template from_cons<F>(g.mat, U);
-    double Wm[NC], Wp[NC];
-#pragma unroll
-    for (int k = 0; k < NC; ++k) { Wm[k] = lane_from_left_or_self(P.W[k]); Wp[k] = lane_from_right_or_self(P.W[k]); }
-    CellStage<M> cs;
-    cell_stage<M, F>(g.mat, g.opt.limiter, s, P, Wm, Wp, cs);
-
-    // left neighbour's data for this lane's left face (lane 0, which has no left neighbour
-    // and whose face is not used, takes its own cell stage: a physical state, so it sends
-    // the wavefront into no fallback branch)
-    Prim<M> Pa;
-#pragma unroll
-    for (int k = 0; k < NC; ++k) Pa.W[k] = Wm[k];
-    Pa.c = lane_from_left(P.c);
-    double Ua[NC];
-#pragma unroll
-    for (int k = 0; k < NC; ++k) Ua[k] = lane_from_left(U[k]);
-    CellStage<M> a;
-    a.fR = lane_from_left_or_self_i(cs.fR ? 1 : 0) != 0;
-    side_from_left<M>(cs.R, a.R);
-
-    FaceOut<M> fo;
-    int fl = face_stage<M, F>(g.mat, g.opt, s, Pa, Ua, a, P, U, cs, fo);
-
-    double Fr[NC];
-#pragma unroll
-    for (int k = 0; k < NC; ++k) Fr[k] = lane_from_right(fo.F[k]);
-    double usr = lane_from_right(fo.ustar);
-    double zsr = M == ALLAIRE ? lane_from_right(fo.zstar) : 0.0;
-
-    // lanes 0 and 63 (halo lanes without a neighbour on one side) update from incomplete
-    // fluxes and are not stored: the repair and the physical-state test skip them
+    // the repair and the physical-state test skip the halo lanes, which are not stored
     double Un[NC];
     bool bad;
-    update_cell<M, F>(g, r, U, fo.F, Fr, fo.ustar, usr, fo.zstar, zsr, Un, bad, store);
+    const int fl = line_step<M, F>(g, s, r, U, store, Un, bad);
     bool rep = false;
     if (M == EULER && store) rep = euler_repair<F>(g.mat, U, Un);
 
     if (face_counted) { tl.slope += fl & 1; tl.evolve += (fl >> 1) & 1; }
     if (store) {
-        double* out = g.nxt + (long long)y * g.pitch + x;
+        // y is wavefront-uniform: each component's row address is a scalar
 #pragma unroll
-        for (int k = 0; k < NC; ++k) __builtin_nontemporal_store(Un[k], out + k * g.plane);
+        for (int k = 0; k < NC; ++k)
+            store_state(Un[k], row_elem(uniform_row(g.nxt + k * g.plane + (long long)y * g.pitch), x));
         tl.repair += rep;
         tl.bad += bad;
         if (LAST) {
@@ -220,14 +305,13 @@ __device__ __forceinline__ void x_line(const GridView& g, double s, double r, in
     }
 }
 
-// The state of the lanes' columns of row y of tile tx.
+// The state of the lanes' columns of row y (wavefront-uniform) of tile tx.
 template <int M>
 __device__ __forceinline__ void x_load(const GridView& g, int tx, int y, double* U)
 {
     const int xs = mirror(tx * XTILE - 2 + (int)(threadIdx.x & 63), g.nx);
-    const double* base = g.cur + (long long)y * g.pitch + xs;
 #pragma unroll
-    for (int k = 0; k < Model<M>::NC; ++k) U[k] = base[k * g.plane];
+    for (int k = 0; k < Model<M>::NC; ++k) U[k] = *row_elem(uniform_row(g.cur + k * g.plane + (long long)y * g.pitch), xs);
 }
 
 // One x-sweep tile: the calling wavefront covers columns [tx*60-2, tx*60+62) and rows
@@ -311,17 +395,20 @@ __device__ __forceinline__ void x_tile_godunov(const GridView& g, double dt, int
         }
         if (neg) U[MN] = -U[MN];
 
+        // lane 0 receives lane 63's state and lane 63 lane 0's flux and z: neither is stored
         double Ua[NC];
 #pragma unroll
-        for (int k = 0; k < NC; ++k) Ua[k] = lane_from_left_or_self(U[k]);
+        for (int k = 0; k < NC; ++k) Ua[k] = rot_from_left(U[k]);
         FaceOut<M> fo;
         godunov_face<M, F>(g.mat, g.opt.wave_speeds, Ua, U, fo);
+        constexpr int NF = M == ALLAIRE ? 5 : NC;  // Allaire's sixth flux component is zero
         double Fr[NC];
 #pragma unroll
-        for (int k = 0; k < NC; ++k) Fr[k] = lane_from_right(fo.F[k]);
-        const double usr = lane_from_right(fo.ustar);
+        for (int k = 0; k < NF; ++k) Fr[k] = rot_from_right(fo.F[k]);
+        if (M == ALLAIRE) Fr[5] = 0.0;
+        const double usr = rot_from_right(fo.ustar);
         const double zL = M == ALLAIRE ? Ua[NC - 1] : 0.0;
-        const double zR = M == ALLAIRE ? lane_from_right_or_self(U[NC - 1]) : 0.0;
+        const double zR = M == ALLAIRE ? rot_from_right(U[NC - 1]) : 0.0;
 
         double Un[NC];
         bool bad;
@@ -331,7 +418,7 @@ __device__ __forceinline__ void x_tile_godunov(const GridView& g, double dt, int
         if (store) {
             double* out = g.nxt + (long long)y * g.pitch + x;
 #pragma unroll
-            for (int k = 0; k < NC; ++k) __builtin_nontemporal_store(Un[k], out + k * g.plane);
+            for (int k = 0; k < NC; ++k) store_state(Un[k], out + k * g.plane);
             tl.repair += rep;
             tl.bad += bad;
             if (LAST) {
@@ -439,7 +526,7 @@ __device__ __forceinline__ void y_tile(const GridView& g, double dt, int bx, int
             if (valid) {
                 double* out = g.nxt + (long long)(c - 1) * g.pitch + x;
 #pragma unroll
-                for (int k = 0; k < NC; ++k) __builtin_nontemporal_store(Un[k], out + k * g.plane);
+                for (int k = 0; k < NC; ++k) store_state(Un[k], out + k * g.plane);
                 tl.repair += rep;
                 tl.bad += bad;
                 if (LAST) {
@@ -511,33 +598,9 @@ __device__ __forceinline__ void y_tile_t(const GridView& g, double dt, int x0, i
         U[MX] = raw[MY];
         U[MY] = -raw[MX];
 
-        Prim<M> P;
-        P.template from_cons<F>(g.mat, U);
-        double Wm[NC], Wp[NC];
-#pragma unroll
-        for (int k = 0; k < NC; ++k) { Wm[k] = lane_from_left_or_self(P.W[k]); Wp[k] = lane_from_right_or_self(P.W[k]); }
-        CellStage<M> cs;
-        cell_stage<M, F>(g.mat, g.opt.limiter, s, P, Wm, Wp, cs);
-        Prim<M> Pa;
-#pragma unroll
-        for (int k = 0; k < NC; ++k) Pa.W[k] = Wm[k];
-        Pa.c = lane_from_left(P.c);
-        double Ua[NC];
-#pragma unroll
-        for (int k = 0; k < NC; ++k) Ua[k] = lane_from_left(U[k]);
-        CellStage<M> a;
-        a.fR = lane_from_left_or_self_i(cs.fR ? 1 : 0) != 0;
-        side_from_left<M>(cs.R, a.R);
-        FaceOut<M> fo;
-        int fl = face_stage<M, F>(g.mat, g.opt, s, Pa, Ua, a, P, U, cs, fo);
-        double Fr[NC];
-#pragma unroll
-        for (int k = 0; k < NC; ++k) Fr[k] = lane_from_right(fo.F[k]);
-        double usr = lane_from_right(fo.ustar);
-        double zsr = M == ALLAIRE ? lane_from_right(fo.zstar) : 0.0;
         double Un[NC];
         bool bad;
-        update_cell<M, F>(g, r, U, fo.F, Fr, fo.ustar, usr, fo.zstar, zsr, Un, bad, store);
+        const int fl = line_step<M, F>(g, s, r, U, store, Un, bad);
         // rotate back (m_x, m_y) -> (-m_y, m_x): E/sourcecode/euler_colsweep.cpp:163-168
         double t = Un[MX];
         Un[MX] = -Un[MY];
@@ -571,7 +634,7 @@ __device__ __forceinline__ void y_tile_t(const GridView& g, double dt, int x0, i
                 if (rr >= 2 && rr < 62 && yy < g.ny) {
                     double* out = g.nxt + (long long)yy * g.pitch + x;
 #pragma unroll
-                    for (int k = 0; k < NC; ++k) __builtin_nontemporal_store(sU[(k * 64 + rr) * PAD + col], out + k * g.plane);
+                    for (int k = 0; k < NC; ++k) store_state(sU[(k * 64 + rr) * PAD + col], out + k * g.plane);
                 }
             }
         }
@@ -622,7 +685,7 @@ __device__ __forceinline__ void y_tile_godunov(const GridView& g, double dt, int
         if (valid) {
             double* out = g.nxt + (long long)(c - 1) * g.pitch + x;
 #pragma unroll
-            for (int k = 0; k < NC; ++k) __builtin_nontemporal_store(Un[k], out + k * g.plane);
+            for (int k = 0; k < NC; ++k) store_state(Un[k], out + k * g.plane);
             tl.repair += rep;
             tl.bad += bad;
             if (LAST) {

@@ -38,8 +38,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M == ALLAIR
     const int b = blockIdx.x + blockIdx.y * gridDim.x;
     const int L = xcd_order(b, gridDim.x * gridDim.y);
     const int tx = L % gridDim.x, ty = L / gridDim.x;
-    const int wave = threadIdx.x >> 6;
-    const int row0 = (ty * 4 + wave) * xrows;
+    const int row0 = (ty * 4 + wave_index()) * xrows;
     double dt;
     if (!sweep_dt(g, dt_arg, dt_factor, use_ctrl, dt)) {
         const int lane = threadIdx.x & 63, x = tx * XTILE - 2 + lane;

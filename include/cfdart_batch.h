@@ -92,10 +92,11 @@ void* cfdart_host_alloc(size_t bytes);
 void cfdart_host_free(void* p);
 
 /* Autotuner measurement: times one sweep of each axis for a grid of this description with a
- * given tile shape, on host_state (format of cfdart.h; NULL: a uniform state), returning the
- * mean kernel time in microseconds over 'reps' launches per axis. */
+ * given tile shape, in the arithmetic 'arith' (CFDART_ARITH_EXACT or CFDART_ARITH_FAST), on
+ * host_state (format of cfdart.h; NULL: a uniform state), returning the mean kernel time in
+ * microseconds over 'reps' launches per axis. */
 int cfdart_time_tile(const cfdart_desc* desc, const double* host_state, int y_strip_rows,
-                     int x_rows_per_wave, int reps, double* us_x, double* us_y);
+                     int x_rows_per_wave, int reps, int arith, double* us_x, double* us_y);
 
 #ifdef __cplusplus
 }
