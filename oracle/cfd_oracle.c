@@ -33,7 +33,13 @@
  * contact preservation; free-stream preservation; x/y transposition invariance; conservation
  * budgets; Toro's exact Riemann solutions (L1 convergence); ghost-fill index tables by
  * brute force; dt by brute force; the Godunov scheme by first-order convergence to the exact
- * Riemann solution and the upwind z update by its closed form on a step. Parity unpinned: the exact Eigen evaluation order of the
+ * Riemann solution and the upwind z update by its closed form on a step; the Euler repair at its
+ * thresholds and by the source of each recovered quantity; the face's M3 and M5 fallbacks as
+ * HLLC of the zero-slope or raw states (Allaire: snapped where the reference snaps); the
+ * splitting schemes composed from single sweeps, on a state where the y sweep is the identity
+ * and against the transposed problem; shock_speed_jump as the exact Rankine-Hugoniot speed.
+ * tests/test_oracle_mutants.py plants a mistake in each of these and checks that its pin fails.
+ * Parity unpinned: the exact Eigen evaluation order of the
  * half-step product (reassociation-level only; no reference binary can be built, Eigen is
  * absent) and the reference's racy multithreaded max in compute_dt.
  */
@@ -492,10 +498,10 @@ void orc_fill_ghosts(const orc_desc* d, double* grid)
 /* ------------------------------------------------------------------ sweeps */
 
 /* Euler positivity repair: E/sourcecode/euler_rowsweep.cpp:47-82 (column variant
- * E/sourcecode/euler_colsweep.cpp:173-208 uses the unrotated old state). */
-static void euler_repair(const orc_desc* d, const double* old, double* fut, long long* counters)
+ * E/sourcecode/euler_colsweep.cpp:78-113 uses the unrotated old state). */
+static int euler_repair(const orc_desc* d, const double* old, double* fut, long long* counters)
 {
-    if (orc_is_physical(d, fut)) return;
+    if (orc_is_physical(d, fut)) return 0;
     if (counters) {
 #pragma omp atomic
         counters[ORC_CNT_REPAIR] += 1;
@@ -513,6 +519,12 @@ static void euler_repair(const orc_desc* d, const double* old, double* fut, long
     double p = orc_pressure(d, rho, e, 0.0);
     double W[4] = {rho, u, v, p};
     orc_p2c(d, W, fut);
+    return 1;
+}
+
+int orc_euler_repair(const orc_desc* d, const double* old, double* fut)
+{
+    return euler_repair(d, old, fut, NULL);
 }
 
 /* One line (row or rotated column): line[l], l in [0, len+2g), the flux stencil runs over it.

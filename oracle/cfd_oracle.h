@@ -87,6 +87,9 @@ void orc_face(const orc_desc* d, const double* s0, const double* s1, const doubl
               const double* s3, double dt, double dx, double* F, double* ustar,
               double* zstar, long long* counters);
 double orc_limit(int limiter, double a, double b);
+/* Euler positivity repair of one updated cell (E/sourcecode/euler_rowsweep.cpp:47-82): fut is
+ * replaced in place when unphysical, from old's density or internal energy; returns 1 if so. */
+int orc_euler_repair(const orc_desc* d, const double* old, double* fut);
 void orc_jacobian(const orc_desc* d, const double* W, double* A);  /* row-major n x n */
 double orc_zupdate(const orc_desc* d, double dx, double dt, double zL, double z, double zR,
                    double usL, double usR, double zsL, double zsR);

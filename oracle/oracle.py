@@ -7,7 +7,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# CFD_ORACLE_LIB names another build of the oracle (the mutation tests load builds with planted
+# mistakes through it, in a separate process)
+LIB_PATH = os.environ.get("CFD_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 
 class OrcDesc(C.Structure):
@@ -49,6 +51,8 @@ def lib():
         L.orc_hllc.argtypes = [D, dp, dp, dp, dp]
         L.orc_face.argtypes = [D, dp, dp, dp, dp, C.c_double, C.c_double, dp, dp, dp, lp]
         L.orc_jacobian.argtypes = [D, dp, dp]
+        L.orc_euler_repair.argtypes = [D, dp, dp]
+        L.orc_euler_repair.restype = C.c_int
         L.orc_limit.argtypes = [C.c_int, C.c_double, C.c_double]
         L.orc_limit.restype = C.c_double
         L.orc_zupdate.argtypes = [D, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
