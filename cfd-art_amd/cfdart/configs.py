@@ -93,17 +93,20 @@ def allaire_1024(seed=1):
 
 
 def batched_mix(n_grids=256, seed=2024, lo=128, hi=1024, euler_frac=0.75):
-    """The batched workload: n_grids independent grids, each Euler (probability euler_frac) or
-    Allaire, with nx and ny drawn independently log-uniform in [lo, hi] (square cells, domain
-    [0,1] x [0,ny/nx]); Euler grids carry 8 seeded discs (radii U[0.03,0.10]), Allaire grids 16
-    seeded air bubbles behind a planar shock. Returns [(desc, state, cfl)]."""
+    """The batched workload: n_grids independent grids, exactly round(euler_frac * n_grids) of them
+    Euler and the rest Allaire (the models in a seeded random order; 192 and 64 of 256), with nx
+    and ny drawn independently log-uniform in [lo, hi] (square cells, domain [0,1] x [0,ny/nx]);
+    Euler grids carry 8 seeded discs (radii U[0.03,0.10]), Allaire grids 16 seeded air bubbles
+    behind a planar shock. Returns [(desc, state, cfl)]."""
     import numpy as np
     rng = np.random.default_rng(seed)
+    n_euler = int(round(euler_frac * n_grids))
+    is_euler = rng.permutation(np.arange(n_grids) < n_euler)
     out = []
     for k in range(n_grids):
         nx = int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
         ny = int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
-        if rng.uniform() < euler_frac:
+        if is_euler[k]:
             d, h = euler_discs(nx, 8, (0.03, 0.10), (0, 0, 0, 0), seed=seed * 1000 + k, ny=ny)
             out.append((d, h, 0.8))
         else:

@@ -24,6 +24,7 @@
 
 namespace cfd {
 void set_last_error(const std::string& msg);  // cfdart_api.hip
+int check_desc(const cfdart_desc* d);         // cfdart_api.hip: cfdart_create's descriptor checks
 }
 
 namespace cfd {
@@ -344,8 +345,9 @@ int admit(cfdart_batch* b, const cfdart_desc* d, const double* host, const cfdar
     if (splitting != CFDART_NAIVE && splitting != CFDART_ALTERNATING)
         return bfail(CFDART_E_ARG, "batched grids use naive or alternating splitting");
     if (max_steps < 1 || !(cfl > 0.0)) return bfail(CFDART_E_ARG, "need max_steps >= 1 and cfl > 0");
-    if (d->model != CFDART_EULER && d->model != CFDART_ALLAIRE) return bfail(CFDART_E_ARG, "bad model");
-    if (d->nx < 2 || d->ny < 2 || d->ng != 2 || !(d->dx > 0) || !(d->dy > 0)) return bfail(CFDART_E_ARG, "bad grid");
+    // every check of cfdart_create, then the batch's own limits (MUSCL-Hancock, ghost width 2)
+    const int rc = cfd::check_desc(d);
+    if (rc) return rc;
     if (d->flux_solver != CFDART_MUSCL_HANCOCK) return bfail(CFDART_E_ARG, "batched grids use MUSCL-Hancock");
     // the y sweep of a batch runs the 16-column transposed tile
     if ((yH != 0 && yH != CFDART_Y_TRANSPOSED) || xrows < 0 || xrows > 256)
@@ -631,19 +633,24 @@ int cfdart_batch_frame(cfdart_batch* b, int slot, double* host_dst)
     const int nc = g.model == cfd::ALLAIRE ? 6 : 4;
     const size_t n = (size_t)g.nx * g.ny * nc;
     if (n > b->ring_cap / 2) {
-        // grow the ring to hold at least two frames of this size (and 128 Mi doubles)
+        // grow the ring to hold four frames of this size (at least 16 Mi doubles, 128 MiB); its
+        // bytes count against the batch's memory budget like the grids' states
+        const size_t cap = std::max<size_t>(4 * n, size_t(1) << 24);
+        const size_t used_without = b->used - sizeof(double) * b->ring_cap;
+        if (b->budget && used_without + sizeof(double) * cap > b->budget)
+            return bfail(CFDART_E_NOMEM, "frame staging ring exceeds the memory budget");
         BCHK(hipStreamSynchronize(b->st));
         BCHK(hipStreamSynchronize(b->copy_st));
         for (FrameRegion& r : b->inflight) (void)hipEventDestroy(r.done);
         b->inflight.clear();
         if (b->d_ring) BCHK(hipFree(b->d_ring));
         b->d_ring = nullptr;
-        b->ring_cap = std::max<size_t>(2 * n, size_t(1) << 27);
+        b->used = used_without;
+        b->ring_cap = 0;
         b->ring_head = 0;
-        if (hipMalloc(&b->d_ring, sizeof(double) * b->ring_cap) != hipSuccess) {
-            b->ring_cap = 0;
-            return bfail(CFDART_E_NOMEM, "frame staging ring");
-        }
+        if (hipMalloc(&b->d_ring, sizeof(double) * cap) != hipSuccess) return bfail(CFDART_E_NOMEM, "frame staging ring");
+        b->ring_cap = cap;
+        b->used += sizeof(double) * cap;
     }
     size_t off = b->ring_head;
     if (off + n > b->ring_cap) off = 0;

@@ -53,7 +53,7 @@ int fail(int code, const std::string& msg)
         if (e_ != hipSuccess) return fail(CFDART_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-int check_desc(const cfdart_desc* d)
+int validate_desc(const cfdart_desc* d)
 {
     if (!d) return fail(CFDART_E_ARG, "null descriptor");
     if (d->model != CFDART_EULER && d->model != CFDART_ALLAIRE) return fail(CFDART_E_ARG, "model must be 0 (Euler) or 1 (Allaire)");
@@ -174,6 +174,8 @@ int step_plan(int splitting, int n, int* axes, double* factors)
 
 namespace cfd {
 void set_last_error(const std::string& msg) { g_err = msg; }
+// The descriptor checks of cfdart_create, shared with batch admission.
+int check_desc(const cfdart_desc* d) { return validate_desc(d); }
 }  // namespace cfd
 
 extern "C" {
@@ -191,7 +193,7 @@ int cfdart_create(const cfdart_desc* desc, cfdart_grid** out)
 {
     if (!out) return fail(CFDART_E_ARG, "null output pointer");
     *out = nullptr;
-    int rc = check_desc(desc);
+    int rc = validate_desc(desc);
     if (rc) return rc;
     if (cfdart_device_count() < 1) return fail(CFDART_E_NODEV, "no HIP device");
     cfdart_grid* g = new cfdart_grid();
@@ -525,7 +527,10 @@ int cfdart_stats(cfdart_grid* g, long long* counters)
 int cfdart_sync(cfdart_grid* g)
 {
     if (!g) return fail(CFDART_E_ARG, "null handle");
+    int err = 0;
+    HIPCHK(hipMemcpyAsync(&err, &g->ctrl->error, sizeof(err), hipMemcpyDeviceToHost, g->st));
     HIPCHK(hipStreamSynchronize(g->st));
+    if (err) return fail(CFDART_E_UNPHYS, "an Allaire state failed the physical-state test");
     return 0;
 }
 

@@ -186,18 +186,22 @@ __global__ void mass_record_kernel(const Ctrl* c, const double* partials, int nb
     hist[2] = m2 * dx * dy;
 }
 
-// Start of the time loop: dt for (n, t) from a fresh reduction, and the 'while (t < T)' test.
+// Start of the time loop: dt for (n, t) from a fresh reduction, and the 'while (t < T)' test. A
+// grid whose Allaire state has failed the physical-state test (where the reference's assert
+// aborts the run) takes no further steps.
 __global__ __launch_bounds__(64) void init_dt_kernel(Ctrl* c, double dx, double dy)
 {
     double mu, mv;
     fold_maxima_wave(c, mu, mv);  // the whole wavefront
     if (threadIdx.x != 0) return;
-    c->active = c->t < c->T;
+    c->active = c->t < c->T && !c->error;
     if (c->active) c->dt = dt_from_maxima(c, mu, mv, c->n, c->t, dx, dy);
 }
 
 // End of a step (E/headers/simulation.hpp:68-70): t += dt, n++, snap to T, loop test, and the
-// next step's dt from the maxima that the step's last sweep accumulated.
+// next step's dt from the maxima that the step's last sweep accumulated. The loop ends after a
+// step in which an Allaire state failed the physical-state test (the reference's assert aborts
+// there), as it does for a grid in a batch.
 __global__ __launch_bounds__(64) void finalize_step_kernel(Ctrl* c, double dx, double dy)
 {
     if (!c->active) return;  // the same value in every lane
@@ -209,7 +213,7 @@ __global__ __launch_bounds__(64) void finalize_step_kernel(Ctrl* c, double dx, d
     if (fabs(t - c->T) < 1e-12) t = c->T;
     c->t = t;
     c->n = n;
-    c->active = t < c->T;
+    c->active = t < c->T && !c->error;
     if (c->active) c->dt = dt_from_maxima(c, mu, mv, n, t, dx, dy);
 }
 

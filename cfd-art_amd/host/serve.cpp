@@ -172,8 +172,7 @@ int cfdart_serve_batched(const cfdart_request* reqs, int n, const cfdart_serve_o
     std::deque<int> waiting;
     int next = 0, done = 0, live = 0;
     long long live_cells = 0;
-    size_t held = 0;
-    std::vector<size_t> slot_bytes(o->capacity, 0);
+    size_t held = 0;  // device bytes the batch counts against the budget (states and frame ring)
     const Clock::time_point t0 = Clock::now();
 
     auto fail_all = [&](int code) {
@@ -213,10 +212,7 @@ int cfdart_serve_batched(const cfdart_request* reqs, int n, const cfdart_serve_o
             waiting.pop_front();
             owner[slot] = r;
             out[r].admitted = seconds_since(t0);
-            size_t bytes = 0;
-            cfdart_batch_info(b, nullptr, &bytes, nullptr);
-            slot_bytes[slot] = bytes - held;
-            held = bytes;
+            cfdart_batch_info(b, nullptr, &held, nullptr);
             ++live;
             live_cells += cells;
         }
@@ -243,11 +239,18 @@ int cfdart_serve_batched(const cfdart_request* reqs, int n, const cfdart_serve_o
                 cfdart_bubble_params bp;
                 double cfl;
                 cfdart_request_setup(&reqs[r], &d, &dp, &bp, &cfl);
+                // a frame that finds no room (host ring smaller than the frame, or a device staging
+                // ring the memory budget cannot hold) is skipped; the grid keeps running
                 double* dst = ring.take(frame_bytes(d));
                 if (dst) {
-                    if ((rc = cfdart_batch_frame(b, s, dst)) != 0) return fail_all(rc);
-                    sum->frames += 1;
-                    sum->frame_bytes += (double)frame_bytes(d);
+                    rc = cfdart_batch_frame(b, s, dst);
+                    if (rc && rc != CFDART_E_NOMEM) return fail_all(rc);
+                    if (!rc) {
+                        sum->frames += 1;
+                        sum->frame_bytes += (double)frame_bytes(d);
+                    }
+                    cfdart_batch_info(b, nullptr, &held, nullptr);
+                    sum->peak_state_bytes = std::max(sum->peak_state_bytes, (double)held);
                 }
                 res.frames = steps[s] / o->frame_every;
             }
@@ -259,8 +262,7 @@ int cfdart_serve_batched(const cfdart_request* reqs, int n, const cfdart_serve_o
                 res.service = res.finished - res.admitted;
                 sum->cell_updates += (double)reqs[r].n * reqs[r].n * steps[s];
                 if ((rc = cfdart_batch_remove(b, s)) != 0) return fail_all(rc);
-                held -= slot_bytes[s];
-                slot_bytes[s] = 0;
+                cfdart_batch_info(b, nullptr, &held, nullptr);
                 owner[s] = -1;
                 --live;
                 live_cells -= (long long)reqs[r].n * reqs[r].n;
@@ -293,12 +295,14 @@ int cfdart_serve_one_at_a_time(const cfdart_request* reqs, int n, const cfdart_s
             res.status = rc;
             continue;
         }
-        const Clock::time_point t0 = Clock::now();
+        // the grid's buffers are allocated before and freed after the timed service, as a server
+        // that pools its buffers would (the batch likewise keeps its pool across requests)
         cfdart_grid* g = nullptr;
         if ((rc = cfdart_create(&d, &g)) != 0) {
             res.status = rc;
             continue;
         }
+        const Clock::time_point t0 = Clock::now();
         rc = d.model == CFDART_EULER ? cfdart_init_random_discs(g, &dp) : cfdart_init_shocked_bubbles(g, &bp);
         if (!rc) rc = cfdart_set_arith(g, o->arith);
         const int m = d.model;
@@ -325,10 +329,10 @@ int cfdart_serve_one_at_a_time(const cfdart_request* reqs, int n, const cfdart_s
         }
         if (!rc) rc = cfdart_checksum(g, &res.checksum);
         if (!rc) rc = cfdart_sync(g);
+        res.service = seconds_since(t0);
         cfdart_destroy(g);
         res.status = rc;
         res.steps_done = taken;
-        res.service = seconds_since(t0);
         res.admitted = std::max(clock, reqs[r].arrival);
         res.finished = res.admitted + res.service;
         clock = res.finished;

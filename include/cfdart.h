@@ -99,7 +99,11 @@ int cfdart_step(cfdart_grid* g, int splitting, int n, double dt);
 /* The time loop of simulation::run_simulation (simulation.hpp:48-61) on the device: from the
  * handle's (t, n) take at most max_steps steps while t < T, each with dt = compute_dt(...).
  * The loop state stays on the device; with t_out/n_out non-null the call synchronises and
- * reports the final t and n. Pass T = INFINITY for a fixed number of steps. */
+ * reports the final t and n. Pass T = INFINITY for a fixed number of steps. A step in which an
+ * Allaire state fails the physical-state test (the reference's assert aborts the run there)
+ * is the loop's last: the call returns CFDART_E_UNPHYS when it synchronises (t_out or n_out
+ * non-null), and otherwise the handle's next cfdart_sync does. A batched grid stops at the
+ * same step (cfdart_batch.h, status flag 2). */
 int cfdart_run(cfdart_grid* g, int splitting, int max_steps, double T, double cfl,
                double* t_out, int* n_out);
 /* As cfdart_run, also recording the mass history of compute_dt (E/sourcecode/euler_timestep.cpp:
@@ -114,6 +118,8 @@ int cfdart_run_mass(cfdart_grid* g, int splitting, int max_steps, double T, doub
 int cfdart_set_time(cfdart_grid* g, double t, int n);
 /* Counters accumulated since upload (see CFDART_CNT_*), 4 values; synchronous. */
 int cfdart_stats(cfdart_grid* g, long long* counters);
+/* Waits for the handle's stream; returns CFDART_E_UNPHYS once an Allaire state of the grid has
+ * failed the physical-state test since its last upload or initialisation. */
 int cfdart_sync(cfdart_grid* g);
 /* The HIP stream (hipStream_t) the handle's kernels run on. */
 void* cfdart_stream(cfdart_grid* g);

@@ -75,14 +75,18 @@ int cfdart_batch_set_tile(cfdart_batch* b, int slot, int y_strip_rows, int x_row
 /* Arithmetic of every grid of the batch, as cfdart_set_arith (default CFDART_ARITH_EXACT); takes
  * effect from the next step. No reference counterpart. */
 int cfdart_batch_set_arith(cfdart_batch* b, int mode);
-/* Live grids, device bytes held by grid state, tiles in the work-list (each may be NULL; asking
- * for the tile count rebuilds a work-list left stale by admissions or removals). */
+/* Live grids, device bytes counted against the memory budget (grid states and the frame
+ * staging ring), tiles in the work-list (each may be NULL; asking for the tile count rebuilds a
+ * work-list left stale by admissions or removals). */
 int cfdart_batch_info(cfdart_batch* b, int* live, size_t* bytes, int* tiles);
 /* Frame snapshot: device-to-device copy of a grid's state into a staging ring in device memory
  * on the batch stream, then an asynchronous copy to host_dst (pinned memory, at least the
  * state's interior size: ncomp * nx * ny doubles, SoA planes) on the copy stream, so stepping
  * continues while frames drain. The batch stream waits only when the ring wraps onto a frame
- * still being copied. Completion: cfdart_batch_sync or cfdart_batch_frames_done. */
+ * still being copied. The ring is allocated on the first call and regrown to four frames of a
+ * larger request (at least 128 MiB); its bytes count against the memory budget, and a ring that
+ * would exceed the budget returns CFDART_E_NOMEM. Completion: cfdart_batch_sync or
+ * cfdart_batch_frames_done. */
 int cfdart_batch_frame(cfdart_batch* b, int slot, double* host_dst);
 int cfdart_batch_frames_done(cfdart_batch* b);
 void* cfdart_batch_stream(cfdart_batch* b);

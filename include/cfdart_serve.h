@@ -88,7 +88,8 @@ typedef struct cfdart_serve_summary {
     long long frames;
     long long chunks;      /* batched: scheduling rounds */
     int max_live;          /* batched: most grids live at once */
-    double peak_state_bytes; /* batched: most grid-state bytes held at once */
+    double peak_state_bytes; /* batched: most device bytes held at once (grid states and the
+                                frame staging ring, as cfdart_batch_info counts them) */
 } cfdart_serve_summary;
 
 /* Continuous batching over reqs (sorted by arrival). out[i] belongs to reqs[i]. */
@@ -96,9 +97,10 @@ int cfdart_serve_batched(const cfdart_request* reqs, int n, const cfdart_serve_o
                          cfdart_serve_summary* sum);
 /* The same requests one at a time with the plain single-grid solver (cfdart.h): create, generate
  * the state on the device, run with the same frames, checksum, destroy. Requests are solved
- * back to back, each timed on its own (service); admitted/finished are the first-come-first-
- * served schedule of those service times against the arrival times, i.e. what a server that
- * solves one request at a time would observe. */
+ * back to back, each timed on its own (service: state generation to checksum; the grid's
+ * allocation and release fall outside it, as for a server that pools its buffers);
+ * admitted/finished are the first-come-first-served schedule of those service times against the
+ * arrival times, i.e. what a server that solves one request at a time would observe. */
 int cfdart_serve_one_at_a_time(const cfdart_request* reqs, int n, const cfdart_serve_opts* o,
                                cfdart_request_result* out, cfdart_serve_summary* sum);
 

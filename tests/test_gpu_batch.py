@@ -95,3 +95,75 @@ def test_batch_admission_budget(gpu):
     b.remove(1)
     assert b.try_add(d, h, max_steps=5) == 1
     assert b.info()[0] == 3
+
+
+def test_unphysical_allaire_stops_alone_and_batched(gpu):
+    """An Allaire bubble state run at CFL 2.5 (beyond the scheme's stability limit) fails the
+    physical-state test during one of its first steps; the oracle counts the failed cells of that
+    step. The reference's assert would abort there: the lone run ends after that step and returns
+    CFDART_E_UNPHYS (synchronously, and at the next sync of an asynchronous run), the same grid in
+    a batch stops at the same step with status flag 2, and every copy holds the oracle's state
+    after that step, bitwise."""
+    d, h = cfdart.configs.allaire_bubbles(48, 3, seed=9)
+    ora = oracle.Sim(d, h)
+    k = 0
+    while ora.counters()[3] == 0:
+        ora.run(cfdart.ALTERNATING, 1, cfl=2.5)
+        k += 1
+        assert k < 30
+    want = interior(ora.grid)
+
+    g = cfdart.Grid(d)
+    g.upload(h)
+    with pytest.raises(cfdart.CfdartError) as e:
+        g.run(cfdart.ALTERNATING, 30, cfl=2.5)
+    assert e.value.code == -4
+    assert bitwise_equal(interior(g.download()), want)
+    assert g.stats()[3] == ora.counters()[3]
+    lone_sum = g.checksum()
+    g.close()
+
+    g = cfdart.Grid(d)
+    g.upload(h)
+    g.run(cfdart.ALTERNATING, 30, cfl=2.5, sync=False)
+    with pytest.raises(cfdart.CfdartError) as e:
+        g.sync()
+    assert e.value.code == -4
+    assert bitwise_equal(interior(g.download()), want)
+    g.close()
+
+    b = cfdart.Batch(4)
+    s = b.add(d, h, splitting=cfdart.ALTERNATING, max_steps=30, cfl=2.5)
+    b.step(30)
+    b.sync()
+    steps, flags = b.poll(4)
+    assert steps[s] == k and flags[s] == 2
+    assert bitwise_equal(interior(b.download(s)), want)
+    assert b.checksum(s) == lone_sum
+    b.close()
+
+
+def test_batch_refuses_what_create_refuses(gpu):
+    """Batch admission applies cfdart_create's descriptor checks (gamma > 1, wall codes,
+    limiter and wave-speed options, PVRS on Euler, the z clip range, the grid size limit)."""
+    de, he = cfdart.configs.euler_discs(32, 2, (0.05, 0.1), (0, 0, 0, 0), seed=1)
+    da, ha = cfdart.configs.allaire_bubbles(32, 2, seed=1)
+    bad = [(de, he, "gamma1", 1.0), (de, he, "limiter", 7), (de, he, "wave_speeds", 1), (da, ha, "wave_speeds", 3),
+           (da, ha, "gamma2", 0.5), (da, ha, "z_lo", 1.0), (da, ha, "z_hi", 1.5)]
+    b = cfdart.Batch(4)
+    for d0, h, field, value in bad:
+        d = type(d0).from_buffer_copy(d0)
+        setattr(d, field, value)
+        with pytest.raises(cfdart.CfdartError) as e:
+            cfdart.Grid(d)
+        assert e.value.code == -1, field
+        with pytest.raises(cfdart.CfdartError) as e:
+            b.add(d, h, max_steps=2)
+        assert e.value.code == -1, field
+    d = type(de).from_buffer_copy(de)
+    d.bc[2] = 2
+    with pytest.raises(cfdart.CfdartError) as e:
+        b.add(d, he, max_steps=2)
+    assert e.value.code == -1
+    assert b.info()[0] == 0
+    b.close()
