@@ -8,6 +8,6 @@ from ._lib import (  # noqa: F401
     CfdartError, Desc, DiscParams, BubbleParams, Grid, Batch, time_tile, lib, library_path,
     EULER, ALLAIRE, TRANSMISSIVE, REFLECTIVE, MINMOD, VANLEER, GARRICK, PVRS,
     NAIVE, STRANG, ALTERNATING, MUSCL_HANCOCK, GODUNOV, Y_TRANSPOSED, Y_TRANSPOSED_NARROW, ARITH_EXACT, ARITH_FAST, ncomp, make_desc, random_discs, shocked_bubbles, planar,
-    prim_to_cons, device_count, state_checksum,
+    prim_to_cons, device_count, state_checksum, test_case,
 )
 from . import autotune, configs, serve  # noqa: F401

@@ -91,6 +91,8 @@ def lib():
         L.cfdart_ic_shocked_bubbles.argtypes = [P(Desc), P(BubbleParams), dp]
         L.cfdart_ic_planar.argtypes = [P(Desc), dp, dp, C.c_double, C.c_double, C.c_double, dp]
         L.cfdart_prim_to_cons.argtypes = [P(Desc), dp, dp]
+        L.cfdart_test_case.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_char_p, P(Desc), dp,
+                                       C.c_size_t, P(C.c_double)]
         L.cfdart_set_tile.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.cfdart_set_arith.argtypes = [C.c_void_p, C.c_int]
         L.cfdart_init_random_discs.argtypes = [C.c_void_p, P(DiscParams)]
@@ -168,6 +170,18 @@ def make_desc(model=EULER, nx=200, ny=200, x0=0.0, y0=0.0, lx=1.0, ly=1.0, gamma
 
 def empty_state(d):
     return np.zeros((d.ny + 2 * d.ng, d.nx + 2 * d.ng, ncomp(d.model)), dtype=np.float64)
+
+
+def test_case(problem, name, nx, ny, flux_solver="MUSCL"):
+    """A named test case of the settings-file driver (cfdart_test_case): (desc, state, T)."""
+    d = Desc()
+    T = C.c_double()
+    _check(lib().cfdart_test_case(problem.encode(), name.encode(), nx, ny, flux_solver.encode(), C.byref(d), None,
+                                  0, C.byref(T)))
+    h = empty_state(d)
+    _check(lib().cfdart_test_case(problem.encode(), name.encode(), nx, ny, flux_solver.encode(), C.byref(d),
+                                  _dptr(h), h.size, C.byref(T)))
+    return d, h, T.value
 
 
 def random_discs(d, p):

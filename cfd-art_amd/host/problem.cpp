@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <fstream>
+#include <functional>
 #include <iostream>
 #include <random>
 #include <sstream>
@@ -62,6 +63,9 @@ namespace {
 int ncomp(const cfdart_desc& d) { return d.model == CFDART_ALLAIRE ? 6 : 4; }
 double ccx(const cfdart_desc& d, int j) { return d.x0 + double(j - d.ng) * d.dx + 0.5 * d.dx; }
 double ccy(const cfdart_desc& d, int i) { return d.y0 + double(i - d.ng) * d.dy + 0.5 * d.dy; }
+
+// eos::specific_ie of one stiffened gas (stiffened_gas_eos.hpp:73-76)
+double sg_specific_ie(double gamma, double pinf, double p, double rho) { return (p + gamma * pinf) / ((gamma - 1.0) * rho); }
 
 std::vector<double> p2c(const cfdart_desc& d, std::vector<double> W)
 {
@@ -124,6 +128,37 @@ struct Field {
                 for (int k = 0; k < nc; ++k) at(i, j)[k] += frac * Uin[k];
             }
     }
+    // set_halfspace_IC (allaire_diffuse_io.cpp:312-330): U where a x + b y <= c at the cell centre
+    void set_halfspace(const std::vector<double>& U, double a, double b, double c)
+    {
+        for (int i = 0; i < H; ++i)
+            for (int j = 0; j < W; ++j)
+                if (ccx(d, j) * a + ccy(d, i) * b <= c)
+                    for (int k = 0; k < nc; ++k) at(i, j)[k] = U[k];
+    }
+    // Fraction of a cell's N x N midpoint samples (the reference's sampling) where inside(x, y)
+    template <class In> double sample_frac(int i, int j, int N, const In& inside)
+    {
+        int count = 0;
+        const double delx = d.dx / N, dely = d.dy / N;
+        const double blx = ccx(d, j) - 0.5 * d.dx, bly = ccy(d, i) - 0.5 * d.dy;
+        for (int a = 0; a < N; ++a)
+            for (int b = 0; b < N; ++b)
+                if (inside(blx + (a + 0.5) * delx, bly + (b + 0.5) * dely)) count++;
+        return double(count) / (N * N);
+    }
+    // A two-fluid cell from its volume fraction and the fluids' densities and specific internal
+    // energies (allaire_diffuse_io.cpp:697-702)
+    void set_mixture(int i, int j, double z, double rho1, double rho2, double u, double v, double e1, double e2)
+    {
+        double* U = at(i, j);
+        U[0] = z * rho1;
+        U[1] = (1.0 - z) * rho2;
+        U[2] = u * (z * rho1 + (1.0 - z) * rho2);
+        U[3] = v * (z * rho1 + (1.0 - z) * rho2);
+        U[4] = z * rho1 * e1 + (1.0 - z) * rho2 * e2 + 0.5 * (z * rho1 + (1.0 - z) * rho2) * (u * u + v * v);
+        U[5] = z;
+    }
     // set_circular_IC (euler_io.cpp:392-433): primitive blend of inside/outside states
     void set_circular(const std::vector<double>& Win, const std::vector<double>& Wout, double cx, double cy,
                       double R, int N = 10)
@@ -149,27 +184,11 @@ struct Field {
 
 int bc_code(const std::string& s) { return s == "reflective" ? CFDART_REFLECTIVE : CFDART_TRANSMISSIVE; }
 
-}  // namespace
-
-GpuProblem::~GpuProblem()
+// The grid, initial state (the host format of cfdart.h, ghost frame included) and end time of
+// a test case: set_parameters and set_ICs of euler_io.cpp:13-234,436-760 and
+// allaire_diffuse_io.cpp:13-310,400-990.
+std::string build_case(const Settings& sf, cfdart_desc& d, std::vector<double>& host, double& T)
 {
-    if (g_) cfdart_destroy(g_);
-}
-
-std::string GpuProblem::set_ICs(const Settings& sf)
-{
-    sf_ = sf;
-    outname_ = sf.basename;
-    cfl_ = sf.CFL;
-    // the flux solver and its ghost width (euler_io.cpp:446-457, allaire_diffuse_io.cpp:412-423)
-    if (sf.flux_solver != "MUSCL" && sf.flux_solver != "Godunov") return "flux_solver must be MUSCL or Godunov";
-    if (sf.riemann_solver != "HLLC") return "riemann_solver must be HLLC";
-    if (sf.splitting_scheme == "naive_splitting") split_ = CFDART_NAIVE;
-    else if (sf.splitting_scheme == "strang_splitting") split_ = CFDART_STRANG;
-    else if (sf.splitting_scheme == "alternating_splitting") split_ = CFDART_ALTERNATING;
-    else return "unknown splitting_scheme " + sf.splitting_scheme;
-
-    cfdart_desc& d = d_;
     d = cfdart_desc{};
     d.nx = sf.Nx;
     d.ny = sf.Ny;
@@ -181,11 +200,11 @@ std::string GpuProblem::set_ICs(const Settings& sf)
     d.z_lo = 0.0;
     d.z_hi = 1.0;
     std::string bc[4] = {"transmissive", "transmissive", "transmissive", "transmissive"};  // L R B T
-    auto domain = [&](double x0, double y0, double lx, double ly, double T) {
-        d.x0 = x0; d.y0 = y0; d.dx = lx / d.nx; d.dy = ly / d.ny; T_ = T;
+    auto domain = [&](double x0, double y0, double lx, double ly, double t_end) {
+        d.x0 = x0; d.y0 = y0; d.dx = lx / d.nx; d.dy = ly / d.ny; T = t_end;
     };
     const std::string& tc = sf.test_case;
-    host_.assign((size_t)(d.nx + 2 * d.ng) * (d.ny + 2 * d.ng) * (sf.problem == "allaire" ? 6 : 4), 0.0);
+    host.assign((size_t)(d.nx + 2 * d.ng) * (d.ny + 2 * d.ng) * (sf.problem == "allaire" ? 6 : 4), 0.0);
 
     if (sf.problem == "euler") {
         // set_parameters (euler_io.cpp:13-234)
@@ -213,7 +232,22 @@ std::string GpuProblem::set_ICs(const Settings& sf)
         else if (tc == "shocked_bubbles") {
             d.gamma1 = 4.4; d.pinf1 = 6e8;
             domain(0, 0, 1, double(d.ny) / d.nx, 2e-4);
-        } else return "invalid Allaire test_case " + tc;
+        }
+        else if (tc == "circular_explosion") domain(0, 0, 2, 2, 0.25);
+        else if (tc == "underwater_shocked_bubble") { d.gamma1 = 7.15; d.pinf1 = 3309.0; domain(0, 0, 12, 12, 0.05); }
+        else if (tc == "underwater_explosion") { d.gamma2 = 7.15; d.pinf2 = 3.309e8; domain(-5, -5, 10, 10, 0.005); }
+        else if (tc == "underwater_explosion_modified") {
+            d.gamma2 = 7.15; d.pinf2 = 3.309e8;
+            domain(-5, -5, 10, 10, 0.010);
+            bc[2] = bc[3] = "reflective";
+        }
+        else if (tc == "shocked_helium_bubble") { d.gamma2 = 1.667; domain(0, 0, 325, 89, 280.0); bc[2] = bc[3] = "reflective"; }
+        else if (tc == "shocked_R22_bubble") { d.gamma2 = 1.249; domain(0, 0, 0.445, 0.089, 0.001080); bc[2] = bc[3] = "reflective"; }
+        else if (tc == "shocked_SF6") { d.gamma2 = 1.076; domain(0, 0, 0.45, 0.2, 0.25); bc[1] = bc[2] = bc[3] = "reflective"; }
+        else if (tc == "RMI_SF6") { d.gamma2 = 1.093; domain(0, 0, 4.0, 0.5, 10.0); bc[2] = bc[3] = "reflective"; }
+        else if (tc == "tin_air_implosion") { d.gamma1 = 3.27; d.pinf1 = 149500.0; domain(0, -25, 25, 50, 0.08); bc[0] = "reflective"; }
+        else if (tc == "TSTM") { d.gamma1 = 1.5; domain(0, 0, 7, 3, 8.0); }
+        else return "invalid Allaire test_case " + tc;
         if (sf.z_clip >= 0.0) { d.z_lo = sf.z_clip; d.z_hi = 1.0 - sf.z_clip; }
     } else {
         return "problem must be euler or allaire";
@@ -221,10 +255,10 @@ std::string GpuProblem::set_ICs(const Settings& sf)
     for (int s = 0; s < 4; ++s)
         if (!sf.bc[s].empty()) bc[s] = sf.bc[s];
     for (int s = 0; s < 4; ++s) d.bc[s] = bc_code(bc[s]);
-    if (sf.T > 0.0) T_ = sf.T;
-    if (sf.num_steps >= 0) T_ = INFINITY;  // fixed step count
+    if (sf.T > 0.0) T = sf.T;
+    if (sf.num_steps >= 0) T = INFINITY;  // fixed step count
 
-    Field f(d, host_);
+    Field f(d, host);
     using V = std::vector<double>;
     if (d.model == CFDART_EULER) {
         V bg = {0.125, 0.0, 0.0, 0.1};
@@ -301,7 +335,7 @@ std::string GpuProblem::set_ICs(const Settings& sf)
             p.cx_lo = 0.1; p.cx_hi = 0.9; p.cy_lo = 0.1 * ly; p.cy_hi = 0.9 * ly;
             p.r_lo = 0.03; p.r_hi = 0.10; p.p_lo = 5.0; p.p_hi = 20.0; p.rho_lo = 0.5; p.rho_hi = 2.0;
             p.ambient[0] = 1.0; p.ambient[1] = 0.0; p.ambient[2] = 0.0; p.ambient[3] = 1.0;
-            if (cfdart_ic_random_discs(&d, &p, host_.data())) return "random_discs failed";
+            if (cfdart_ic_random_discs(&d, &p, host.data())) return "random_discs failed";
         }
     } else {
         // allaire_diffuse_io.cpp:467-564 (planar states; the _y variants keep the x velocity)
@@ -316,8 +350,73 @@ std::string GpuProblem::set_ICs(const Settings& sf)
         else if (tc.rfind("TTC5_", 0) == 0) { L = {5.99924, 0.0, 19.5975, 0.0, 460.894, 1.0}; R = {5.99242, 0.0, -6.19633, 0.0, 46.0950, 1.0}; }
         if (!L.empty()) {
             bool ydir = tc.find("_y") != std::string::npos;
-            if (cfdart_ic_planar(&d, L.data(), R.data(), ydir ? 0.0 : 1.0, ydir ? 1.0 : 0.0, cut, host_.data()))
+            if (cfdart_ic_planar(&d, L.data(), R.data(), ydir ? 0.0 : 1.0, ydir ? 1.0 : 0.0, cut, host.data()))
                 return "planar IC failed";
+        } else if (tc == "circular_explosion") {  // allaire_diffuse_io.cpp:565-574
+            f.set_circular({1.0, 0.0, 0.0, 0.0, 1.0, 1.0}, {0.0, 0.125, 0.0, 0.0, 0.1, 0.0}, 1.0, 1.0, 0.4);
+        } else if (tc == "shocked_helium_bubble") {  // :575-588
+            f.set_circular({0.0, 0.138, 0.0, 0.0, 1.0, 0.0}, {1.0, 0.0, 0.0, 0.0, 1.0, 1.0}, 175.0, 44.5, 25.0);
+            f.set_halfspace(p2c(d, {1.3764, 0.0, -0.394, 0.0, 1.5698, 1.0}), -1.0, 0.0, -225.0);
+        } else if (tc == "shocked_R22_bubble") {  // :589-602
+            f.set_circular({0.0, 3.863, 0.0, 0.0, 1.01325e5, 0.0}, {1.225, 0.0, 0.0, 0.0, 1.01325e5, 1.0}, 0.225, 0.0445,
+                           0.025, 25);
+            f.set_halfspace(p2c(d, {1.686, 0.0, -113.5, 0.0, 1.59e5, 1.0}), -1.0, 0.0, -0.275);
+        } else if (tc == "underwater_shocked_bubble") {  // :603-616
+            f.set_circular({0.0, 0.0012, 0.0, 0.0, 1.0, 0.0}, {1.0, 0.0, 0.0, 0.0, 1.0, 1.0}, 6.0, 6.0, 3.0);
+            f.set_halfspace(p2c(d, {1.31, 0.0, 67.32, 0.0, 19000.0, 1.0}), 1.0, 0.0, 2.4);
+        } else if (tc == "underwater_explosion" || tc == "underwater_explosion_modified") {  // :617-630
+            f.set_circular({1270.0, 0.0, 0.0, 0.0, 8.29e8, 1.0}, {0.0, 1000.0, 0.0, 0.0, 1.0e5, 0.0}, 0.0, 0.0, 1.0);
+            f.set_halfspace(p2c(d, {1.0, 0.0, 0.0, 0.0, 1.0e5, 1.0}), 0.0, -1.0, -2.5);
+        } else if (tc == "shocked_SF6" || tc == "TSTM" || tc == "RMI_SF6") {
+            // :631-704, :705-803, :804-876: fluid 1 pre- or post-shock by the cell centre, fluid 2
+            // at rest, z = 1 - the fraction of 10 x 10 samples inside fluid 2's region
+            struct Gas { double rho, p, u; };
+            Gas pre, post, g2;
+            std::function<bool(double, double)> in2;
+            std::function<bool(double)> shocked;
+            if (tc == "shocked_SF6") {
+                pre = {1.153, 9.6856, 0.0}; post = {1.6672, 16.3256, 1.33273}; g2 = {5.805, 9.6856, 0.0};
+                in2 = [](double x, double y) { return x >= 0.1 && x <= 0.25 && y >= 0.0 && y <= 0.1; };
+                shocked = [](double cx) { return !(cx > 0.05); };
+            } else if (tc == "TSTM") {
+                pre = {0.125, 0.1, 0.0}; post = {1.0, 1.0, 0.0}; g2 = {1.0, 0.1, 0.0};
+                in2 = [](double x, double y) { return x >= 1.0 && x <= 8.0 && y >= -1.0 && y <= 1.5; };
+                shocked = [](double cx) { return !(cx > 1.0); };
+            } else {
+                pre = {1.0, 1.0, 0.0}; post = {1.411, 1.628, -0.39}; g2 = {5.04, 1.0, 0.0};
+                const double pi = std::atan(1.0) * 4.0;
+                in2 = [pi](double x, double y) { return 2.9 - 0.2 * std::sin(2.0 * pi * (y + 0.25)) > x; };
+                shocked = [](double cx) { return !(cx < 3.2); };
+            }
+            const double e_pre = sg_specific_ie(d.gamma1, d.pinf1, pre.p, pre.rho);
+            const double e_post = sg_specific_ie(d.gamma1, d.pinf1, post.p, post.rho);
+            const double e2 = sg_specific_ie(d.gamma2, d.pinf2, g2.p, g2.rho);
+            for (int i = 0; i < f.H; ++i)
+                for (int j = 0; j < f.W; ++j) {
+                    const bool s = shocked(ccx(d, j));
+                    const double rho1 = s ? post.rho : pre.rho, u = s ? post.u : pre.u, e1 = s ? e_post : e_pre;
+                    const double z = 1.0 - f.sample_frac(i, j, 10, in2);
+                    f.set_mixture(i, j, z, rho1, g2.rho, u, 0.0, e1, e2);
+                }
+        } else if (tc == "tin_air_implosion") {  // :877-990
+            const double e_pre = sg_specific_ie(d.gamma1, d.pinf1, 1.0, 7.28);
+            const double e_post = sg_specific_ie(d.gamma1, d.pinf1, 1000000.0, 11.84);
+            const double e2 = sg_specific_ie(d.gamma2, d.pinf2, 1.0, 0.001);
+            auto in_tin = [](double x, double y) { return std::sqrt(x * x + y * y) <= 24.0; };
+            auto in_interface = [](double x, double y) {
+                double theta = std::atan2(y, x);
+                theta -= std::atan(1) * 2;
+                const double r = 20.0 + 0.4 * std::cos(22 * theta) + 0.4 * std::cos(17 * theta) + 0.3 * std::cos(29 * theta);
+                return std::sqrt(x * x + y * y) <= r;
+            };
+            for (int i = 0; i < f.H; ++i)
+                for (int j = 0; j < f.W; ++j) {
+                    const double in = f.sample_frac(i, j, 10, in_tin);
+                    const double rho1 = in * 7.28 + (1.0 - in) * 11.84;
+                    const double e1 = in * e_pre + (1.0 - in) * e_post;
+                    const double z = 1.0 - f.sample_frac(i, j, 10, in_interface);
+                    f.set_mixture(i, j, z, rho1, 0.001, 0.0, 0.0, e1, e2);
+                }
         } else {  // shocked_bubbles
             double ze = sf.z_clip >= 0.0 ? sf.z_clip : 1e-6, zw = 1.0 - ze, ly = double(d.ny) / d.nx;
             d.z_lo = ze; d.z_hi = zw;
@@ -331,9 +430,35 @@ std::string GpuProblem::set_ICs(const Settings& sf)
             const double post[6] = {zw * 1323.65, (1 - zw) * 1.0, 681.58, 0.0, 1e9, zw};
             for (int k = 0; k < 6; ++k) { p.water[k] = water[k]; p.air[k] = air[k]; p.postshock[k] = post[k]; }
             p.shock_x = 0.1;
-            if (cfdart_ic_shocked_bubbles(&d, &p, host_.data())) return "shocked_bubbles failed";
+            if (cfdart_ic_shocked_bubbles(&d, &p, host.data())) return "shocked_bubbles failed";
         }
     }
+    return "";
+}
+
+}  // namespace
+
+GpuProblem::~GpuProblem()
+{
+    if (g_) cfdart_destroy(g_);
+}
+
+std::string GpuProblem::set_ICs(const Settings& sf)
+{
+    sf_ = sf;
+    outname_ = sf.basename;
+    cfl_ = sf.CFL;
+    // the flux solver and its ghost width (euler_io.cpp:446-457, allaire_diffuse_io.cpp:412-423)
+    if (sf.flux_solver != "MUSCL" && sf.flux_solver != "Godunov") return "flux_solver must be MUSCL or Godunov";
+    if (sf.riemann_solver != "HLLC") return "riemann_solver must be HLLC";
+    if (sf.splitting_scheme == "naive_splitting") split_ = CFDART_NAIVE;
+    else if (sf.splitting_scheme == "strang_splitting") split_ = CFDART_STRANG;
+    else if (sf.splitting_scheme == "alternating_splitting") split_ = CFDART_ALTERNATING;
+    else return "unknown splitting_scheme " + sf.splitting_scheme;
+
+    cfdart_desc& d = d_;
+    const std::string err = build_case(sf, d, host_, T_);
+    if (!err.empty()) return err;
 
     int rc = cfdart_create(&d, &g_);
     if (rc) return std::string("cfdart_create: ") + cfdart_last_error();
@@ -501,3 +626,45 @@ int run_simulation(const Settings& sf, std::string* err)
 }
 
 }  // namespace cfdhost
+
+namespace cfd {
+void set_last_error(const std::string& msg);  // cfdart_api.hip
+}
+
+extern "C" int cfdart_test_case(const char* problem, const char* test_case, int nx, int ny, const char* flux_solver,
+                                cfdart_desc* d_out, double* host_state, size_t host_len, double* T_out)
+{
+    if (!problem || !test_case || !flux_solver || !d_out || nx < 2 || ny < 2) {
+        cfd::set_last_error("bad argument");
+        return CFDART_E_ARG;
+    }
+    cfdhost::Settings sf;
+    sf.problem = problem;
+    sf.test_case = test_case;
+    sf.flux_solver = flux_solver;
+    sf.Nx = nx;
+    sf.Ny = ny;
+    if (sf.problem == "allaire") sf.wave_speeds = "garrick";
+    if (sf.flux_solver != "MUSCL" && sf.flux_solver != "Godunov") {
+        cfd::set_last_error("flux_solver must be MUSCL or Godunov");
+        return CFDART_E_ARG;
+    }
+    cfdart_desc d;
+    std::vector<double> host;
+    double T = 0.0;
+    const std::string err = cfdhost::build_case(sf, d, host, T);
+    if (!err.empty()) {
+        cfd::set_last_error(err);
+        return CFDART_E_ARG;
+    }
+    *d_out = d;
+    if (T_out) *T_out = T;
+    if (host_state) {
+        if (host_len < host.size()) {
+            cfd::set_last_error("host_state holds fewer than (nx+2ng)*(ny+2ng)*ncomp doubles");
+            return CFDART_E_ARG;
+        }
+        std::copy(host.begin(), host.end(), host_state);
+    }
+    return 0;
+}

@@ -7,6 +7,8 @@
 #ifndef CFDART_HOST_H
 #define CFDART_HOST_H
 
+#include <stddef.h>
+
 #include "cfdart.h"
 
 #ifdef __cplusplus
@@ -62,6 +64,17 @@ int cfdart_init_shocked_bubbles(cfdart_grid* g, const cfdart_bubble_params* p);
 /* Primitive -> conserved of the model (euler_equations/.../headers/misc_euler.hpp:82-97,
  * allaire_2d/headers/misc.hpp:149-163). */
 int cfdart_prim_to_cons(const cfdart_desc* d, const double* W, double* U);
+
+/* A named test case of the settings-file driver (set_parameters + set_ICs,
+ * E/sourcecode/euler_io.cpp:13-234,436-760 and A/sourcecode/allaire_diffuse_io.cpp:13-310,
+ * 400-990; problem "euler" or "allaire", flux_solver "MUSCL" or "Godunov"): its grid descriptor
+ * with the case's defaults (limiter minmod, Garrick wave speeds, z clip [0, 1], the case's walls),
+ * its end time T, and, when host_state is non-null, its initial state in the format of cfdart.h
+ * ((nx+2ng)*(ny+2ng)*ncomp doubles, ghost frame included; host_len: the buffer's length in
+ * doubles). Host only: no device is needed. CFDART_E_ARG names an unknown case in
+ * cfdart_last_error. */
+int cfdart_test_case(const char* problem, const char* test_case, int nx, int ny, const char* flux_solver,
+                     cfdart_desc* d_out, double* host_state, size_t host_len, double* T_out);
 
 #ifdef __cplusplus
 }
