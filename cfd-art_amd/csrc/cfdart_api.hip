@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -79,28 +80,34 @@ int validate_desc(const cfdart_desc* d)
     return 0;
 }
 
-cfd::GridView view(const cfdart_grid* g)
+cfd::GridView make_view(const cfdart_desc& d, const double* cur, double* nxt, int pitch, long long plane, int arith,
+                        cfd::Ctrl* ctrl)
 {
     cfd::GridView v;
-    v.cur = g->buf[g->cur];
-    v.nxt = g->buf[1 - g->cur];
-    v.plane = g->plane;
-    v.pitch = g->pitch;
-    v.nx = g->d.nx;
-    v.ny = g->d.ny;
-    for (int i = 0; i < 4; ++i) v.bc[i] = g->d.bc[i];
-    v.mat = cfd::make_material(g->d.gamma1, g->d.pinf1, g->d.gamma2, g->d.pinf2);
-    v.opt.limiter = g->d.limiter;
-    v.opt.wave_speeds = g->d.wave_speeds;
-    v.opt.z_lo = g->d.z_lo;
-    v.opt.z_hi = g->d.z_hi;
-    v.opt.godunov = g->d.flux_solver == CFDART_GODUNOV ? 1 : 0;
-    v.opt.fast = g->arith == CFDART_ARITH_FAST ? 1 : 0;
-    v.dx = g->d.dx;
-    v.dy = g->d.dy;
-    v.dx_half_col = g->d.colsweep_uses_dx ? g->d.dx : g->d.dy;
-    v.ctrl = g->ctrl;
+    v.cur = cur;
+    v.nxt = nxt;
+    v.plane = plane;
+    v.pitch = pitch;
+    v.nx = d.nx;
+    v.ny = d.ny;
+    for (int i = 0; i < 4; ++i) v.bc[i] = d.bc[i];
+    v.mat = cfd::make_material(d.gamma1, d.pinf1, d.gamma2, d.pinf2);
+    v.opt.limiter = d.limiter;
+    v.opt.wave_speeds = d.wave_speeds;
+    v.opt.z_lo = d.z_lo;
+    v.opt.z_hi = d.z_hi;
+    v.opt.godunov = d.flux_solver == CFDART_GODUNOV ? 1 : 0;
+    v.opt.fast = arith == CFDART_ARITH_FAST ? 1 : 0;
+    v.dx = d.dx;
+    v.dy = d.dy;
+    v.dx_half_col = d.colsweep_uses_dx ? d.dx : d.dy;
+    v.ctrl = ctrl;
     return v;
+}
+
+cfd::GridView view(const cfdart_grid* g)
+{
+    return make_view(g->d, g->buf[g->cur], g->buf[1 - g->cur], g->pitch, g->plane, g->arith, g->ctrl);
 }
 
 hipEvent_t next_event(cfdart_grid* g)
@@ -580,3 +587,110 @@ int cfdart_set_arith(cfdart_grid* g, int mode)
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ the time loop without allocation
+
+namespace {
+
+// The loop's control block and first-step reduction partials for cfdart_run_noalloc: a file-scope
+// device array, one per GPU, held until the process ends. Every call on a GPU uses that GPU's
+// array and re-initialises it first, so calls that overlap on one GPU race.
+constexpr size_t NOALLOC_SCRATCH_BYTES = 32 * 1024;
+constexpr size_t NOALLOC_CTRL_BYTES = (sizeof(cfd::Ctrl) + 255) / 256 * 256;
+static_assert(sizeof(cfd::Ctrl) % 8 == 0, "the control block is cleared as 64-bit words");
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_addr;  // per device: the array's device address, looked up once
+
+// The array's address on the current device (host code may not use the array's name as an
+// address: each device holds its own copy at its own address).
+int scratch_address(void** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(CFDART_E_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((size_t)dev >= g_scratch_addr.size()) g_scratch_addr.resize((size_t)dev + 1, nullptr);
+    if (!g_scratch_addr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(CFDART_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_addr[dev] = p;
+    }
+    *out = g_scratch_addr[dev];
+    return 0;
+}
+
+// A fresh loop state: every word of the control block zero (counters, wave-speed slots, error
+// flag), then t, n, T and CFL. A kernel rather than a memset, so that a captured graph replays it.
+__global__ void noalloc_init_kernel(cfd::Ctrl* c, double t, int n, double T, double cfl)
+{
+    unsigned long long* w = reinterpret_cast<unsigned long long*>(c);
+    for (int i = threadIdx.x; i < (int)(sizeof(cfd::Ctrl) / 8); i += blockDim.x) w[i] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        c->t = t;
+        c->n = n;
+        c->T = T;
+        c->cfl = cfl;
+    }
+}
+
+}  // namespace
+
+extern "C" int cfdart_run_noalloc(const cfdart_desc* d, double* state, double* work, int pitch, int arith,
+                                  int splitting, int max_steps, double T, double cfl, double t0, int n0, void* stream,
+                                  double* t_out, int* n_out)
+{
+    int rc = validate_desc(d);
+    if (rc) return rc;
+    if (!state || !work || state == work) return fail(CFDART_E_ARG, "state and work must be two distinct device buffers");
+    if (pitch < d->nx) return fail(CFDART_E_ARG, "pitch must be at least nx");
+    if (arith != CFDART_ARITH_EXACT && arith != CFDART_ARITH_FAST) return fail(CFDART_E_ARG, "bad arithmetic mode");
+    if (splitting < 0 || splitting > 2 || max_steps < 0 || !(cfl > 0.0) || n0 < 0) return fail(CFDART_E_ARG, "bad argument");
+    const long long plane = (long long)pitch * d->ny;
+    const int nc = d->model == CFDART_ALLAIRE ? 6 : 4;
+    const int nred = cfd::reduce_blocks(make_view(*d, state, work, pitch, plane, arith, nullptr));
+    const size_t need = NOALLOC_CTRL_BYTES + sizeof(double) * 2 * (size_t)nred;
+    if (need > NOALLOC_SCRATCH_BYTES)
+        return fail(CFDART_E_ARG, "the loop needs " + std::to_string(need) + " bytes of scratch, beyond the " +
+                                      std::to_string(NOALLOC_SCRATCH_BYTES) + " the library holds");
+    void* base = nullptr;
+    if ((rc = scratch_address(&base)) != 0) return rc;
+    cfd::Ctrl* ctrl = static_cast<cfd::Ctrl*>(base);
+    double* partials = reinterpret_cast<double*>(static_cast<char*>(base) + NOALLOC_CTRL_BYTES);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+
+    hipLaunchKernelGGL(noalloc_init_kernel, dim3(1), dim3(64), 0, st, ctrl, t0, n0, T, cfl);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = cfd::launch_reduce_dt(make_view(*d, state, work, pitch, plane, arith, ctrl), d->model, partials, st);
+    if (e == hipSuccess) e = cfd::launch_init_dt(ctrl, d->dx, d->dy, st);
+    double* cur = state;
+    double* nxt = work;
+    for (int k = 0; k < max_steps && e == hipSuccess; ++k) {
+        int axes[3];
+        double f[3];
+        const int ns = step_plan(splitting, n0 + k, axes, f);
+        for (int s = 0; s < ns && e == hipSuccess; ++s) {
+            e = cfd::launch_sweep(make_view(*d, cur, nxt, pitch, plane, arith, ctrl), d->model, axes[s], s == ns - 1,
+                                  0.0, f[s], 1, CFDART_Y_TRANSPOSED, 4, st);
+            std::swap(cur, nxt);
+        }
+        if (e == hipSuccess) e = cfd::launch_finalize(ctrl, d->dx, d->dy, st);
+    }
+    // an odd number of sweeps (Strang) leaves the state in the work buffer
+    if (e == hipSuccess && cur != state)
+        e = hipMemcpyAsync(state, cur, sizeof(double) * (size_t)plane * nc, hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return fail(CFDART_E_HIP, std::string("cfdart_run_noalloc: ") + hipGetErrorString(e));
+    if (t_out || n_out) {
+        cfd::Ctrl c;
+        HIPCHK(hipMemcpyAsync(&c, ctrl, sizeof(c), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (t_out) *t_out = c.t;
+        if (n_out) *n_out = c.n;
+        if (c.error) return fail(CFDART_E_UNPHYS, "an Allaire state failed the physical-state test");
+    }
+    return 0;
+}

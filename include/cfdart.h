@@ -154,6 +154,21 @@ int cfdart_set_tile(cfdart_grid* g, int y_strip_rows, int x_rows_per_wave);
  * it is split into cfdart_run calls); the fold of the maxima into dt is exact. Batches
  * (cfdart_batch_set_arith) and the serving loop (cfdart_serve_opts.arith) take the mode too.
  * No reference counterpart (the reference has one arithmetic). */
+/* cfdart_run without a handle and without allocation: no counterpart in the reference. For
+ * callers that cannot allocate device memory (code under HIP graph capture, serving loops and
+ * harnesses that own all memory). The caller passes the state in the library's device layout
+ * (ncomp planes of pitch*ny doubles, interior cells only, rows of pitch >= nx doubles) and a
+ * second buffer of the same size; the loop runs from (t0, n0) on 'stream' (NULL: the null stream)
+ * with the default tile shapes, and the final state ends in 'state'. The loop's control block and
+ * reduction partials live in a 32 KiB device array of the library, one per GPU and held until the
+ * process ends; a grid whose partials would not fit is refused with CFDART_E_ARG (none does: the
+ * need is at most 17.25 KiB). Calls that overlap on one GPU share that array and race. Asynchronous
+ * unless t_out or n_out is non-null (then as cfdart_run, including CFDART_E_UNPHYS). Results are
+ * bitwise those of cfdart_run on a handle in the same arithmetic. */
+int cfdart_run_noalloc(const cfdart_desc* d, double* state, double* work, int pitch, int arith, int splitting,
+                       int max_steps, double T, double cfl, double t0, int n0, void* stream, double* t_out,
+                       int* n_out);
+
 #define CFDART_ARITH_EXACT 0
 #define CFDART_ARITH_FAST 1
 int cfdart_set_arith(cfdart_grid* g, int mode);
