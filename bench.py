@@ -108,7 +108,7 @@ def bench_batched(args):
     import cfdart
 
     cases = cfdart.configs.batched_mix(args.grids)
-    mode = cfdart.ARITH_FAST if args.arith == "fast" else cfdart.ARITH_EXACT
+    mode = cfdart.ARITH_EXACT if args.arith == "exact" else cfdart.ARITH_FAST
     table = cfdart.autotune.tune(classes=(128, 256, 512, 1024), log=log, arith=mode)
     steps = args.batch_steps
     cells = [c[0].nx * c[0].ny for c in cases]
@@ -198,7 +198,7 @@ def bench_served(args):
     def opts(cap):
         return serve.opts(mem_budget=200 * 10**9, capacity=8192, chunk_steps=10, frame_every=50,
                           frame_host_bytes=2 << 30, realtime=True, tile=tile, plain_tile=plain, live_cells=cap,
-                          arith=cfdart.ARITH_FAST if args.arith == "fast" else cfdart.ARITH_EXACT)
+                          arith=cfdart.ARITH_EXACT if args.arith == "exact" else cfdart.ARITH_FAST)
 
     o = opts(live_cap)
     rb, sb = serve.serve_batched(reqs, o)
@@ -250,9 +250,10 @@ def main():
                     help="oracle steps of euler_2048 on one thread, about 10 s (0: skip)")
     ap.add_argument("--strip-rows", type=int, default=0)
     ap.add_argument("--tile", default="", help="Y_STRIP_ROWS,X_ROWS_PER_WAVE: fixed tile shape, no autotuning")
-    ap.add_argument("--arith", default="fast", choices=["fast", "exact"],
+    ap.add_argument("--arith", default="fast", choices=["fast", "exact", "fast-only"],
                     help="single-grid workloads: fast = time the exact and the fast arithmetic and report the fast "
-                         "one with its accuracy against the exact state; exact = the exact arithmetic only. Batched "
+                         "one with its accuracy against the exact state; exact = the exact arithmetic only; "
+                         "fast-only = the fast arithmetic alone (for profiling). Batched "
                          "and served workloads run the batch and its one-at-a-time baseline in the given arithmetic")
     ap.add_argument("--workload", default="euler_2048",
                     choices=["euler_2048", "allaire_1024", "euler_200", "euler_slab", "godunov_2048", "batched",
@@ -271,14 +272,16 @@ def main():
     import cfdart
 
     if args.copy_probe:
-        import torch
-        n = 2048 * 2048 * 4
-        a = torch.ones(n, dtype=torch.float64, device="cuda")
-        b = torch.empty_like(a)
-        for _ in range(5):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": 5}))
+        # the library's plain copy kernel over the 2048^2 Euler state (4 planes of 2048 x 2048)
+        d, h = cfdart.configs.euler_2048()
+        g = cfdart.Grid(d)
+        g.upload(h)
+        _check = cfdart._lib._check
+        _check(cfdart.lib().cfdart_copy_state(g.h, 5))
+        n = 4 * d.ny * ((d.nx + 31) // 32 * 32)
+        g.close()
+        print(json.dumps({"copy_probe_kernel": "copy_kernel", "copy_probe_bytes_read": n * 8,
+                          "copy_probe_bytes_written": n * 8, "launches": 5}))
         return
 
     if cfdart.device_count() < 1:
@@ -326,7 +329,7 @@ def main():
     d, h = single["cfg"]()
     cfl = single["cfl"]
     cells = d.nx * d.ny
-    modes = ["exact", "fast"] if args.arith == "fast" else ["exact"]
+    modes = {"fast": ["exact", "fast"], "exact": ["exact"], "fast-only": ["fast"]}[args.arith]
     arith_of = {"exact": cfdart.ARITH_EXACT, "fast": cfdart.ARITH_FAST}
     # each arithmetic runs at the tile shape tuned in that arithmetic
     tiles = {}
@@ -431,7 +434,7 @@ def main():
         "tile": {"y_strip_rows": tuned["y_strip_rows"], "x_rows_per_wave": tuned["x_rows_per_wave"]},
         "fallback_counters": head["counters"],
     }
-    if modes[-1] == "fast":
+    if modes[-1] == "fast" and "exact" in res:
         ex = res["exact"]
         ex_dom = ex["kernels"][dom]["gbs"]
         out["exact_arith"] = {"value": ex["value"], "ms_per_step": 1e3 * ex["el"] / ex["steps"],

@@ -91,6 +91,7 @@ def lib():
         L.cfdart_ic_shocked_bubbles.argtypes = [P(Desc), P(BubbleParams), dp]
         L.cfdart_ic_planar.argtypes = [P(Desc), dp, dp, C.c_double, C.c_double, C.c_double, dp]
         L.cfdart_prim_to_cons.argtypes = [P(Desc), dp, dp]
+        L.cfdart_copy_state.argtypes = [C.c_void_p, C.c_int]
         L.cfdart_run_noalloc.argtypes = [P(Desc), C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
                                          C.c_double, C.c_double, C.c_double, C.c_int, C.c_void_p, P(C.c_double),
                                          P(C.c_int)]

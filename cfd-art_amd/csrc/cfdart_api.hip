@@ -336,6 +336,15 @@ int cfdart_schlieren(cfdart_grid* g, double* host_dst, double* max_grad)
     return 0;
 }
 
+int cfdart_copy_state(cfdart_grid* g, int reps)
+{
+    if (!g || reps < 1) return fail(CFDART_E_ARG, "bad handle or repetition count");
+    for (int i = 0; i < reps; ++i)
+        HIPCHK(cfd::launch_copy(g->buf[g->cur], g->buf[1 - g->cur], g->plane * g->nc, g->st));
+    HIPCHK(hipStreamSynchronize(g->st));
+    return 0;
+}
+
 int cfdart_frame(cfdart_grid* g, double* host_dst)
 {
     if (!g || !host_dst) return fail(CFDART_E_ARG, "null argument");

@@ -74,4 +74,16 @@ hipError_t launch_schlieren(const GridView& g, int model, double* out, unsigned 
     return hipGetLastError();
 }
 
+__global__ __launch_bounds__(256) void copy_kernel(const double* __restrict__ src, double* __restrict__ dst, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+hipError_t launch_copy(const double* src, double* dst, long long n, hipStream_t st)
+{
+    hipLaunchKernelGGL(copy_kernel, dim3(4096), dim3(256), 0, st, src, dst, n);
+    return hipGetLastError();
+}
+
 }  // namespace cfd

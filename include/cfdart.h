@@ -190,6 +190,12 @@ int cfdart_schlieren(cfdart_grid* g, double* host_dst, double* max_grad);
  * reference's output (euler_io.cpp:1142-1261) would write, without its ghost frame. */
 int cfdart_frame(cfdart_grid* g, double* host_dst);
 
+/* Measurement: copies the grid's state buffer (ncomp * pitch * ny doubles) into its other buffer
+ * 'reps' times with a plain copy kernel, then synchronises: the kernel of known traffic against
+ * which hardware byte counters (rocprofv3 FETCH_SIZE, WRITE_SIZE) are calibrated. The state is
+ * unchanged. No reference counterpart. */
+int cfdart_copy_state(cfdart_grid* g, int reps);
+
 #ifdef __cplusplus
 }
 #endif
