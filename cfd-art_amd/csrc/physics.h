@@ -13,8 +13,8 @@
 //   face           E/headers/flux_solver_MUSCLhancock.hpp:128-227, A/headers/flux_solver_MUSCLhancock.hpp:272-376
 //   HLLC           E/headers/riemann_solver_HLLC.hpp:50-145, A/headers/riemann_solver_HLLC.hpp:49-150
 //   z update       A/headers/zupdate.hpp:58-73
-// std::min(a,b) is (b<a)?b:a and std::max(a,b) is (a<b)?b:a; fmin/fmax are never used because
-// they differ on signed zeros and NaN.
+// std::min(a,b) is (b<a)?b:a and std::max(a,b) is (a<b)?b:a; the exact arithmetic never uses
+// fmin/fmax, which differ on signed zeros and NaN.
 //
 // Restructuring relative to the reference (values unchanged): the reference evaluates each
 // face from a 4-cell stencil, so each cell's primitives, slope, half-step states and their
@@ -93,6 +93,25 @@ template <bool F = false> CFD_HD double dsqrt(double x)
 #endif
 }
 
+// sqrt(x / rho), the form of every sound and shock speed here. F = false: dsqrt(ddiv(x, rho)).
+// F = true (device): x * y with y = v_rsq_f64(x rho) refined by one Newton step,
+// x y = sqrt(x/rho) (x >= 0, rho > 0) -- one reciprocal square root in place of a reciprocal and
+// a square root. x = +0 and x = +inf give themselves, as the root of x / rho does for rho > 0.
+template <bool F = false> CFD_HD double sqrt_ratio(double x, double rho)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (F) {
+        const double a = x * rho;
+        const double y = __builtin_amdgcn_rsq(a);
+        const double r = __builtin_fma(-(a * y), y, 1.0);  // 1 - a y^2
+        const double c = x * y;
+        const double cr = __builtin_fma(c * 0.5, r, c);
+        return __builtin_amdgcn_class(x, 0x260) ? x : cr;
+    }
+#endif
+    return dsqrt<F>(ddiv<F>(x, rho));
+}
+
 template <int M> struct Model;
 template <> struct Model<EULER> {
     static constexpr int NC = 4;
@@ -109,10 +128,14 @@ template <> struct Model<ALLAIRE> {
 // sub-expressions the reference recomputes on every call. kmin and epsmin are lower bounds of
 // the Allaire mixture k and eps over z in [0, 1], used only by the conservative positivity
 // screens below (never in a value that is stored or compared bitwise).
+// The fast arithmetic evaluates the mixture's eps(z) = z e1 + (1-z) e2 as e2 + z (e1 - e2) and
+// k(z) eps(z) = z q1 + (1-z) q2 as q2 + z (q1 - q2), one fma each, with ei = 1/(gi - 1) and
+// qi = gi pinfi ei held as fe2 = e2, fe12 = e1 - e2, fk2 = q2, fk12 = q1 - q2.
 struct Material {
     double g1, pinf1, g2, pinf2;
     double g1m1, g2m1, g1pinf1;
     double kmin, epsmin;
+    double fe2, fe12, fk2, fk12;
 };
 
 CFD_HD Material make_material(double g1, double pinf1, double g2, double pinf2)
@@ -124,6 +147,10 @@ CFD_HD Material make_material(double g1, double pinf1, double g2, double pinf2)
     m.kmin = k1 < k2 ? k1 : k2;
     double e1 = 1.0 / (g1 - 1.0), e2 = 1.0 / (g2 - 1.0);
     m.epsmin = (e1 < e2 ? e1 : e2) * (1.0 - 1e-12);
+    m.fe2 = e2;
+    m.fe12 = e1 - e2;
+    m.fk2 = k2 * e2;
+    m.fk12 = k1 * e1 - k2 * e2;
     return m;
 }
 
@@ -145,6 +172,11 @@ struct Mix { double eps, k; };
 template <bool F = false> CFD_HD Mix mixture(const Material& m, double z)
 {
     Mix r;
+    if constexpr (F) {
+        r.eps = __builtin_fma(z, m.fe12, m.fe2);
+        r.k = ddiv<F>(__builtin_fma(z, m.fk12, m.fk2), r.eps);
+        return r;
+    }
     r.eps = ddiv<F>(z, m.g1m1) + ddiv<F>(1.0 - z, m.g2m1);
     r.k = ddiv<F>(ddiv<F>(z * m.g1 * m.pinf1, m.g1m1) + ddiv<F>((1.0 - z) * m.g2 * m.pinf2, m.g2m1), r.eps);
     return r;
@@ -155,17 +187,17 @@ template <> struct Eos<EULER> {
     // E/headers/stiffened_gas_eos.hpp:57-70
     CFD_HD static double pressure(const Material& m, double rho, double e) { return m.g1m1 * rho * e - m.g1pinf1; }
     template <bool F = false> CFD_HD static double specific_ie(const Material& m, double rho, double p) { return ddiv<F>(p + m.g1pinf1, m.g1m1 * rho); }
-    template <bool F = false> CFD_HD static double soundspeed(const Material& m, double rho, double p) { return dsqrt<F>(ddiv<F>(m.g1 * (p + m.pinf1), rho)); }
+    template <bool F = false> CFD_HD static double soundspeed(const Material& m, double rho, double p) { return sqrt_ratio<F>(m.g1 * (p + m.pinf1), rho); }
 };
 struct MixEos {
     // A/headers/mixturemodel.hpp:18-52
     template <bool F = false> CFD_HD static double pressure(const Mix& x, double rho, double e) { return ddiv<F>(rho * e, x.eps) - x.k; }
     template <bool F = false> CFD_HD static double specific_ie(const Mix& x, double rho, double p) { return ddiv<F>((p + x.k) * x.eps, rho); }
-    template <bool F = false> CFD_HD static double soundspeed(const Mix& x, double rho, double p) { return dsqrt<F>(ddiv<F>(x.k + p * (ddiv<F>(1.0, x.eps) + 1.0), rho)); }
+    template <bool F = false> CFD_HD static double soundspeed(const Mix& x, double rho, double p) { return sqrt_ratio<F>(x.k + p * (ddiv<F>(1.0, x.eps) + 1.0), rho); }
     template <bool F = false> CFD_HD static double shock_jump(const Mix& x, double p, double rho, double pstar)
     {
         double p_avg = 0.5 * (pstar + p);
-        return dsqrt<F>(ddiv<F>(pstar + x.k + ddiv<F>(p_avg, x.eps), rho));
+        return sqrt_ratio<F>(pstar + x.k + ddiv<F>(p_avg, x.eps), rho);
     }
 };
 
@@ -317,8 +349,18 @@ template <bool F = false> CFD_HD bool allaire_physical_screened(const double* U)
 
 // ---------------------------------------------------------------- limiter
 
+// The fast arithmetic forms minmod as max(0, min(a, b)) + min(0, max(a, b)) (one term is zero:
+// the same value, without the data-dependent selects; a zero may carry the other sign) and van
+// Leer's 2ab/(a+b) from the product a*b its sign test forms.
 template <bool F = false> CFD_HD double limit(int limiter, double a, double b)
 {
+    if constexpr (F) {
+        if (limiter == 1) {
+            const double t = a * b;
+            return t > 0.0 ? ddiv<F>(t + t, a + b) : 0.0;
+        }
+        return fmax(0.0, fmin(a, b)) + fmin(0.0, fmax(a, b));
+    }
     if (limiter == 1) return (a * b > 0.0) ? ddiv<F>(2.0 * a * b, a + b) : 0.0;
     if (b > 0.0) return smax(0.0, smin(a, b));
     return smin(0.0, smax(a, b));
@@ -340,7 +382,8 @@ template <int M> struct Side {
 
 // The conserved state of a fast-arithmetic Euler evolved side from its rho, u, v, p; returns
 // rho e = (p + gamma pinf)/(gamma - 1), without the division by rho that specific_ie's product
-// undoes. A side sent to a neighbouring lane is rebuilt there by the same function.
+// undoes. Such a side carries its primitives only; U is formed by this function where a face
+// needs it, from the same four values wherever that is.
 CFD_HD double euler_fast_cons(const Material& m, Side<EULER>& s)
 {
     const double rho = s.rho, u = s.u, v = s.v;
@@ -406,10 +449,11 @@ template <bool F = false> CFD_HD void side_from_cons_mix(const Mix& x, Side<ALLA
 template <bool F = false> CFD_HD void side_from_prim_(const Material& m, const double* T, Side<EULER>& s)
 {
     if constexpr (F) {
+        // U is formed (euler_fast_cons) only where a face needs it; rho e = (p + gamma pinf)/(gamma - 1)
+        // has the sign of p + gamma pinf
         s.rho = T[0]; s.u = T[1]; s.v = T[2]; s.p = T[3];
-        const double rhoe = euler_fast_cons(m, s);
         s.c = Eos<EULER>::soundspeed<F>(m, s.rho, s.p);
-        s.ok = s.rho > 0.0 && rhoe >= 0.0;
+        s.ok = s.rho > 0.0 && s.p + m.g1pinf1 >= 0.0;
         return;
     }
     p2c<EULER, F>(m, T, s.U);
@@ -423,6 +467,23 @@ template <bool F = false> CFD_HD void side_from_prim_(const Material& m, const d
     Mix x = mixture<F>(m, T[5]);
     // rho e; the fast arithmetic without the division by rho that specific_ie's product undoes
     double rhoe = F ? (T[4] + x.k) * x.eps : rho * MixEos::specific_ie<F>(x, rho, T[4]);
+    if constexpr (F) {
+        // no component within 1e-12 of a snapping bound: snapping is the identity (the
+        // general path below reaches the same result with the snapping's tests)
+        if (!(fabs(T[0]) < 1e-12) && !(fabs(T[1]) < 1e-12) && !(fabs(T[5]) < 1e-12) && !(fabs(T[5] - 1.0) < 1e-12)) {
+            s.U[0] = T[0];
+            s.U[1] = T[1];
+            s.U[2] = rho * T[2];
+            s.U[3] = rho * T[3];
+            s.U[4] = rhoe + 0.5 * rho * (T[2] * T[2] + T[3] * T[3]);
+            s.U[5] = T[5];
+            s.rho = rho; s.u = T[2]; s.p = T[4];
+            s.c = MixEos::soundspeed<F>(x, rho, T[4]);
+            s.eps = x.eps; s.k = x.k;
+            s.ok = T[0] >= 0.0 && T[1] >= 0.0 && rho > 0.0 && rhoe >= 0.0 && T[5] >= 0.0 && T[5] <= 1.0;
+            return;
+        }
+    }
     s.U[0] = T[0];
     s.U[1] = T[1];
     s.U[2] = rho * T[2];
@@ -446,6 +507,13 @@ template <bool F = false> CFD_HD void side_from_prim_(const Material& m, const d
 template <int M, bool F = false> CFD_HD void side_from_prim(const Material& m, const double* T, Side<M>& s)
 {
     side_from_prim_<F>(m, T, s);
+}
+
+// The conserved state of a side made by side_from_prim: a fast-arithmetic Euler side holds only
+// its primitives until a face needs U (every other side already has it).
+template <int M, bool F = false> CFD_HD void form_cons(const Material& m, Side<M>& s)
+{
+    if constexpr (F && M == EULER) (void)euler_fast_cons(m, s);
 }
 
 // The first-order fallback state (M5 failed): the raw stencil cell, unsnapped.
@@ -575,6 +643,7 @@ CFD_HD void unsloped_side(const Material& m, double s, const Prim<M>& P, bool ri
     jac_times<F>(P.W, P.c, s, del, tmp, (Model<M>*)nullptr);
     for (int k = 0; k < NC; ++k) T[k] = (right ? P.W[k] + 0.5 * del[k] : P.W[k] - 0.5 * del[k]) - tmp[k];
     side_from_prim<M, F>(m, T, out);
+    form_cons<M, F>(m, out);
 }
 
 // ---------------------------------------------------------------- HLLC
@@ -586,7 +655,9 @@ template <int M> struct FaceOut {
 
 // HLLC flux from two prepared sides (E/headers/riemann_solver_HLLC.hpp:69-144,
 // A/headers/riemann_solver_HLLC.hpp:70-149) and z* (A/headers/flux_solver_MUSCLhancock.hpp:375).
-template <int M, bool F = false>
+// PRIM_U (fast-arithmetic Euler sides that carry only rho, u, v, p): the upwind side's U is
+// formed here from its primitives, after the upwind choice, instead of being read from the side.
+template <int M, bool F = false, bool PRIM_U = false>
 CFD_HD void hllc(const Material& m, int wave_speeds, const Side<M>& L, const Side<M>& R, FaceOut<M>& o)
 {
     constexpr int NC = Model<M>::NC;
@@ -622,27 +693,40 @@ CFD_HD void hllc(const Material& m, int wave_speeds, const Side<M>& L, const Sid
         const Side<M>& W = left ? L : R;
         const double S = left ? SL : SR, rho = left ? rhoL : rhoR, u = left ? uL : uR, p = left ? pL : pR;
         const double mS = left ? smin(0.0, SL) : smax(0.0, SR);
-        const double f = ddiv<F>(S - u, S - us);
-        const double e = W.U[NC == 4 ? 3 : 4] + (us - u) * (rho * us + ddiv<F>(p, S - u));
+        double Uw[NC];
+        if constexpr (PRIM_U && M == EULER) {
+            Side<EULER> t;
+            t.rho = rho; t.u = u; t.v = left ? L.v : R.v; t.p = p;
+            (void)euler_fast_cons(m, t);
+            for (int k = 0; k < NC; ++k) Uw[k] = t.U[k];
+        } else {
+            for (int k = 0; k < NC; ++k) Uw[k] = W.U[k];
+        }
+        // one reciprocal, 1/(S - us): f = (S - u)/(S - us), and the star energy
+        // f (E + (us - u)(rho us + p/(S - u))) as f (E + (us - u) rho us) + (us - u) p/(S - us)
+        const double rd = ddiv<F>(1.0, S - us);
+        const double f = (S - u) * rd;
+        const double dus = us - u, rus = rho * us;
+        const double fe = __builtin_fma(f, __builtin_fma(dus, rus, Uw[NC == 4 ? 3 : 4]), dus * p * rd);
         for (int k = 0; k < NC; ++k) {
             double st, fl;
             if (M == ALLAIRE && k == 5) {
                 o.F[k] = 0.0;
                 continue;
             } else if (k == MN) {
-                st = f * rho * us;
-                fl = W.U[k] * u + p;
+                st = f * rus;
+                fl = Uw[k] * u + p;
             } else if (k == NC - 1 - (M == ALLAIRE ? 1 : 0)) {
-                st = f * e;
-                fl = (W.U[k] + p) * u;
+                st = fe;
+                fl = (Uw[k] + p) * u;
             } else {
-                st = f * W.U[k];
-                fl = W.U[k] * u;
+                st = f * Uw[k];
+                fl = Uw[k] * u;
             }
-            o.F[k] = fl + mS * (st - W.U[k]);
+            o.F[k] = fl + mS * (st - Uw[k]);
         }
         o.ustar = us;
-        o.zstar = M == ALLAIRE ? W.U[NC - 1] : 0.0;
+        o.zstar = M == ALLAIRE ? Uw[NC - 1] : 0.0;
         return;
     }
     double fL = ddiv<F>(SL - uL, SL - us);
@@ -694,6 +778,8 @@ CFD_HD int face_stage(const Material& m, const SchemeOpts& o, double s,
 {
     int flags = 0;
     Side<M> L = a.R, R = b.L;
+    form_cons<M, F>(m, L);
+    form_cons<M, F>(m, R);
     if (!(a.fR && b.fL)) {
         flags |= 1;
         unsloped_side<M, F>(m, s, Pa, true, L);

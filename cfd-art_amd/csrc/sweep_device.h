@@ -32,11 +32,19 @@ __device__ __forceinline__ double rot_from_right(double x)
 // code under it runs with every lane active, so it may move values between lanes).
 __device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
 
+// Lane i receives lane i-1's predicate (lane 0 receives false): the wavefront's lane mask shifted
+// by one lane, in scalar registers. Every lane of the wavefront must be active.
+__device__ __forceinline__ bool from_left_lane(bool p)
+{
+    return __builtin_amdgcn_inverse_ballot_w64(__builtin_amdgcn_ballot_w64(p) << 1);
+}
+
 // The right-hand evolved side of the left neighbour's cell stage (its 'ok' flag travels
 // separately). Only what cannot be rebuilt is moved: rho is U[0] (Euler) or U[0] + U[1]
-// (Allaire) on every side, and a fast-arithmetic Euler side is a function of its rho, u, v, p.
+// (Allaire) on every side, and a fast-arithmetic Euler side is its rho, u, v, p (and c), from
+// which form_cons builds U where a face needs it.
 template <int M, bool F>
-__device__ __forceinline__ void side_from_left(const Material& m, const Side<M>& s, Side<M>& o)
+__device__ __forceinline__ void side_from_left(const Side<M>& s, Side<M>& o)
 {
     if constexpr (F && M == EULER) {
         o.rho = rot_from_left(s.rho);
@@ -44,7 +52,6 @@ __device__ __forceinline__ void side_from_left(const Material& m, const Side<M>&
         o.v = rot_from_left(s.v);
         o.p = rot_from_left(s.p);
         o.c = rot_from_left(s.c);
-        (void)euler_fast_cons(m, o);
     } else {
 #pragma unroll
         for (int k = 0; k < Model<M>::NC; ++k) o.U[k] = rot_from_left(s.U[k]);
@@ -123,17 +130,16 @@ __device__ __forceinline__ double wave_max(double v)
     for (int o = 32; o > 0; o >>= 1) v = smax(v, __shfl_xor(v, o));
     return v;
 }
-__device__ __forceinline__ int wave_sum(int v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
 
 struct Tally {
     double su = 0.0, sv = 0.0;  // running max of |u|+c, |v|+c (NaN never enters: smax keeps the old value)
+    // Wavefront-wide counts, the same in every lane: each increment is the population count of a
+    // ballot, taken where every lane of the wavefront is active (the counting costs scalar
+    // instructions only).
     int slope = 0, evolve = 0, repair = 0, bad = 0;
-    // Wavefront-wide reduction, then lane 0 issues the atomics into a slot of the control block
+    __device__ static void count(int& n, bool p) { n += __builtin_popcountll(__builtin_amdgcn_ballot_w64(p)); }
+    __device__ static void count(int& n, unsigned long long lanes) { n += __builtin_popcountll(lanes); }
+    // Wavefront-wide reduction of the maxima, then lane 0 issues the atomics into a slot of the control block
     // (slot = the workgroup's number times its wavefront count plus the wavefront's). Every lane
     // of the wavefront must call it. There is no workgroup barrier: a barrier's release fence
     // would hold the whole workgroup, and its LDS and wavefront slots, until every global store
@@ -142,7 +148,7 @@ struct Tally {
     {
         double mu = 0.0, mv = 0.0;
         if (last) { mu = wave_max(su); mv = wave_max(sv); }
-        const int a = wave_sum(slope), b = wave_sum(evolve), d = wave_sum(repair), e = wave_sum(bad);
+        const int a = slope, b = evolve, d = repair, e = bad;
         if ((threadIdx.x & 63) == 0) {
             const int sl = (slot * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6)) & (MAX_SLOTS - 1);
             if (last) {
@@ -196,13 +202,14 @@ __device__ __forceinline__ void update_cell(const GridView& g, double r, const d
 // (primitives, limited slopes from its neighbours' primitives, M3 flags, evolved sides) and the
 // face to its left from its left neighbour's right-hand side and its own left-hand side (M3 and M5
 // fallbacks, HLLC), receives the face to its right from the next lane, and updates its cell into
-// Un (bad: the Allaire physical-state test failed, tested where 'checked'). Returns the face's
-// fallback flags (bit 0 slope, bit 1 evolve). A stored cell's result needs its lane's neighbours
+// Un (bad: the Allaire physical-state test failed, tested where 'checked'). slope_fb and
+// evolve_fb: the lanes whose face took a fallback (wavefront-uniform lane masks). A stored cell's result needs its lane's neighbours
 // two lanes either side, so lanes 0, 1, 62 and 63 are halo lanes; lanes 0 and 63 receive values
 // from no neighbour and are never stored, and neither is anything derived from them.
 template <int M, bool F>
-__device__ __forceinline__ int line_step(const GridView& g, double s, double r, const double* U, bool checked,
-                                         double* Un, bool& bad)
+__device__ __forceinline__ void line_step(const GridView& g, double s, double r, const double* U, bool checked,
+                                          double* Un, bool& bad, unsigned long long& slope_lanes,
+                                          unsigned long long& evolve_lanes)
 {
     constexpr int NC = Model<M>::NC;
     Prim<M> P;
@@ -213,24 +220,30 @@ __device__ __forceinline__ int line_step(const GridView& g, double s, double r, 
     CellStage<M> cs;
     cell_stage<M, F>(g.mat, g.opt.limiter, s, P, Wm, Wp, cs);
 
-    // the face's left input: the left neighbour's right-hand side, with its M5 and M3 flags in
-    // one word; its right input: this cell's left-hand side
+    // the face's left input: the left neighbour's right-hand side (its M5 and M3 flags arrive as
+    // shifted lane masks); its right input: this cell's left-hand side
     Side<M> L, R = cs.L;
-    side_from_left<M, F>(g.mat, cs.R, L);
-    const int fw = rot_from_left_i((cs.R.ok ? 1 : 0) | (cs.fR ? 2 : 0));
-    L.ok = (fw & 1) != 0;
-    int flags = 0;
+    side_from_left<M, F>(cs.R, L);
+    L.ok = from_left_lane(cs.R.ok);
     // M3: both slopes of the face are zeroed when either extrapolation is unphysical. The left
     // neighbour's primitives and sound speed are moved (again) only when some lane of the
-    // wavefront needs them, so they need not stay live across the cell stage.
-    const bool slope_fb = !((fw & 2) != 0 && cs.fL);
-    if (wave_any(slope_fb)) {
+    // wavefront needs them, so they need not stay live across the cell stage. Sides that carry
+    // only primitives get their U before any fallback replaces a side.
+    // Only the faces of lanes 2..62 reach a stored cell (lanes 2..61 store; lane i updates from
+    // the faces of lanes i and i+1), so only their fallbacks make the wavefront take the
+    // fallback branches (lane 0, whose left neighbour is no lane, always reports one).
+    constexpr unsigned long long FACES = 0x7FFFFFFFFFFFFFFCull;
+    const bool slope_fb = !(from_left_lane(cs.fR) && cs.fL);
+    slope_lanes = __builtin_amdgcn_ballot_w64(slope_fb) & FACES;
+    const bool any_slope = slope_lanes != 0;
+    if (any_slope) {
+        form_cons<M, F>(g.mat, L);
+        form_cons<M, F>(g.mat, R);
         Prim<M> Pa;
 #pragma unroll
         for (int k = 0; k < NC; ++k) Pa.W[k] = rot_from_left(P.W[k]);
         Pa.c = rot_from_left(P.c);
         if (slope_fb) {
-            flags |= 1;
             unsloped_side<M, F>(g.mat, s, Pa, true, L);
             unsloped_side<M, F>(g.mat, s, P, false, R);
         }
@@ -238,18 +251,25 @@ __device__ __forceinline__ int line_step(const GridView& g, double s, double r, 
     // M5: an unphysical evolved side sends the face to the raw stencil cells; the left
     // neighbour's raw state is moved only when some lane needs it
     const bool evolve_fb = !(L.ok && R.ok);
-    if (wave_any(evolve_fb)) {
+    evolve_lanes = __builtin_amdgcn_ballot_w64(evolve_fb) & FACES;
+    const bool any_evolve = evolve_lanes != 0;
+    if (any_evolve) {
+        if (!any_slope) {
+            form_cons<M, F>(g.mat, L);
+            form_cons<M, F>(g.mat, R);
+        }
         double Ua[NC];
 #pragma unroll
         for (int k = 0; k < NC; ++k) Ua[k] = rot_from_left(U[k]);
         if (evolve_fb) {
-            flags |= 2;
             side_from_raw<M, F>(g.mat, Ua, L);
             side_from_raw<M, F>(g.mat, U, R);
         }
     }
     FaceOut<M> fo;
-    hllc<M, F>(g.mat, g.opt.wave_speeds, L, R, fo);
+    // without a fallback in the wavefront, a fast Euler face forms only its upwind side's U
+    if (F && M == EULER && !any_slope && !any_evolve) hllc<M, F, true>(g.mat, g.opt.wave_speeds, L, R, fo);
+    else hllc<M, F>(g.mat, g.opt.wave_speeds, L, R, fo);
 
     // the face to the right, from the next lane (Allaire's sixth flux component is zero, and the
     // z update replaces that component of the new state: it is not moved)
@@ -261,7 +281,6 @@ __device__ __forceinline__ int line_step(const GridView& g, double s, double r, 
     const double usr = rot_from_right(fo.ustar);
     const double zsr = M == ALLAIRE ? rot_from_right(fo.zstar) : 0.0;
     update_cell<M, F>(g, r, U, fo.F, Fr, fo.ustar, usr, fo.zstar, zsr, Un, bad, checked);
-    return flags;
 }
 
 // ------------------------------------------------------------------ x sweep
@@ -284,18 +303,22 @@ __device__ __forceinline__ void x_line(const GridView& g, double s, double r, in
     // the repair and the physical-state test skip the halo lanes, which are not stored
     double Un[NC];
     bool bad;
-    const int fl = line_step<M, F>(g, s, r, U, store, Un, bad);
+    unsigned long long sfb, efb;
+    line_step<M, F>(g, s, r, U, store, Un, bad, sfb, efb);
     bool rep = false;
     if (M == EULER && store) rep = euler_repair<F>(g.mat, U, Un);
 
-    if (face_counted) { tl.slope += fl & 1; tl.evolve += (fl >> 1) & 1; }
+    // rep and bad hold only on stored lanes
+    const unsigned long long counted = __builtin_amdgcn_ballot_w64(face_counted);
+    Tally::count(tl.slope, sfb & counted);
+    Tally::count(tl.evolve, efb & counted);
+    Tally::count(tl.repair, rep);
+    Tally::count(tl.bad, bad);
     if (store) {
         // y is wavefront-uniform: each component's row address is a scalar
 #pragma unroll
         for (int k = 0; k < NC; ++k)
             store_state(Un[k], row_elem(uniform_row(g.nxt + k * g.plane + (long long)y * g.pitch), x));
-        tl.repair += rep;
-        tl.bad += bad;
         if (LAST) {
             double su, sv;
             wave_speeds<F>(g.mat, Un, su, sv, (Model<M>*)nullptr);
@@ -415,12 +438,12 @@ __device__ __forceinline__ void x_tile_godunov(const GridView& g, double dt, int
         update_cell_godunov<M, F>(g, r, U, zL, zR, fo.F, Fr, fo.ustar, usr, Un, bad, store);
         bool rep = false;
         if (M == EULER && store) rep = euler_repair<F>(g.mat, U, Un);
+        Tally::count(tl.repair, store && rep);
+        Tally::count(tl.bad, store && bad);
         if (store) {
             double* out = g.nxt + (long long)y * g.pitch + x;
 #pragma unroll
             for (int k = 0; k < NC; ++k) store_state(Un[k], out + k * g.plane);
-            tl.repair += rep;
-            tl.bad += bad;
             if (LAST) {
                 double su, sv;
                 wave_speeds<F>(g.mat, Un, su, sv, (Model<M>*)nullptr);
@@ -504,7 +527,9 @@ __device__ __forceinline__ void y_tile(const GridView& g, double dt, int bx, int
         cell_stage<M, F>(g.mat, g.opt.limiter, s, P0, Pm.W, Pp.W, cs);
         FaceOut<M> fo;
         int fl = face_stage<M, F>(g.mat, g.opt, s, Pm, Um, a, P0, U0, cs, fo);
-        if (valid && (c < y1 || c == g.ny)) { tl.slope += fl & 1; tl.evolve += (fl >> 1) & 1; }
+        const bool counted = valid && (c < y1 || c == g.ny);
+        Tally::count(tl.slope, counted && (fl & 1) != 0);
+        Tally::count(tl.evolve, counted && (fl & 2) != 0);
         if (c > y0) {
             // update cell c-1 from faces c-1 (Fm) and c (fo)
             double Un[NC];
@@ -523,12 +548,12 @@ __device__ __forceinline__ void y_tile(const GridView& g, double dt, int bx, int
                 old[MY] = Um[MX];
                 rep = euler_repair<F>(g.mat, old, Un);
             }
+            Tally::count(tl.repair, valid && rep);
+            Tally::count(tl.bad, valid && bad);
             if (valid) {
                 double* out = g.nxt + (long long)(c - 1) * g.pitch + x;
 #pragma unroll
                 for (int k = 0; k < NC; ++k) store_state(Un[k], out + k * g.plane);
-                tl.repair += rep;
-                tl.bad += bad;
                 if (LAST) {
                     double su, sv;
                     wave_speeds<F>(g.mat, Un, su, sv, (Model<M>*)nullptr);
@@ -600,17 +625,21 @@ __device__ __forceinline__ void y_tile_t(const GridView& g, double dt, int x0, i
 
         double Un[NC];
         bool bad;
-        const int fl = line_step<M, F>(g, s, r, U, store, Un, bad);
+        unsigned long long sfb, efb;
+        line_step<M, F>(g, s, r, U, store, Un, bad, sfb, efb);
         // rotate back (m_x, m_y) -> (-m_y, m_x): E/sourcecode/euler_colsweep.cpp:163-168
         double t = Un[MX];
         Un[MX] = -Un[MY];
         Un[MY] = t;
         bool rep = false;
         if (M == EULER && store) rep = euler_repair<F>(g.mat, raw, Un);
-        if (face_row && col_ok) { tl.slope += fl & 1; tl.evolve += (fl >> 1) & 1; }
+        // rep and bad hold only on stored lanes
+        const unsigned long long counted = __builtin_amdgcn_ballot_w64(face_row && col_ok);
+        Tally::count(tl.slope, sfb & counted);
+        Tally::count(tl.evolve, efb & counted);
+        Tally::count(tl.repair, rep);
+        Tally::count(tl.bad, bad);
         if (store) {
-            tl.repair += rep;
-            tl.bad += bad;
             if (LAST) {
                 double su, sv;
                 wave_speeds<F>(g.mat, Un, su, sv, (Model<M>*)nullptr);
@@ -682,12 +711,12 @@ __device__ __forceinline__ void y_tile_godunov(const GridView& g, double dt, int
             old[MY] = U0[MX];
             rep = euler_repair<F>(g.mat, old, Un);
         }
+        Tally::count(tl.repair, valid && rep);
+        Tally::count(tl.bad, valid && bad);
         if (valid) {
             double* out = g.nxt + (long long)(c - 1) * g.pitch + x;
 #pragma unroll
             for (int k = 0; k < NC; ++k) store_state(Un[k], out + k * g.plane);
-            tl.repair += rep;
-            tl.bad += bad;
             if (LAST) {
                 double su, sv;
                 wave_speeds<F>(g.mat, Un, su, sv, (Model<M>*)nullptr);
