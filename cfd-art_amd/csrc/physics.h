@@ -130,12 +130,15 @@ template <> struct Model<ALLAIRE> {
 // screens below (never in a value that is stored or compared bitwise).
 // The fast arithmetic evaluates the mixture's eps(z) = z e1 + (1-z) e2 as e2 + z (e1 - e2) and
 // k(z) eps(z) = z q1 + (1-z) q2 as q2 + z (q1 - q2), one fma each, with ei = 1/(gi - 1) and
-// qi = gi pinfi ei held as fe2 = e2, fe12 = e1 - e2, fk2 = q2, fk12 = q1 - q2.
+// qi = gi pinfi ei held as fe2 = e2, fe12 = e1 - e2, fk2 = q2, fk12 = q1 - q2. Its Euler wave
+// speeds take gamma (p + pinf) = wa w + wb from w = E - (mu^2 + mv^2)/(2 rho), with
+// wa = gamma (gamma - 1) and wb = gamma (pinf - gamma pinf).
 struct Material {
     double g1, pinf1, g2, pinf2;
     double g1m1, g2m1, g1pinf1;
     double kmin, epsmin;
     double fe2, fe12, fk2, fk12;
+    double wa, wb;
 };
 
 CFD_HD Material make_material(double g1, double pinf1, double g2, double pinf2)
@@ -151,6 +154,8 @@ CFD_HD Material make_material(double g1, double pinf1, double g2, double pinf2)
     m.fe12 = e1 - e2;
     m.fk2 = k2 * e2;
     m.fk12 = k1 * e1 - k2 * e2;
+    m.wa = g1 * (g1 - 1.0);
+    m.wb = g1 * (pinf1 - g1 * pinf1);
     return m;
 }
 
@@ -851,8 +856,19 @@ CFD_HD void godunov_face(const Material& m, int wave_speeds, const double* Ua, c
 
 // Wave speed |u|+c, |v|+c of a final (unrotated) state: compute_dt's per-cell work
 // (E/sourcecode/euler_timestep.cpp:16-24, A/sourcecode/allaire_diffuse_timestep.cpp:17-26).
+// The fast arithmetic forms the Euler speeds from one reciprocal of rho: |u| + c = |mu| (1/rho) + c
+// with c = sqrt(gamma (p + pinf) / rho) and gamma (p + pinf) = wa w + wb (the pressure's
+// gamma pinf and the sound speed's pinf folded into one constant).
 template <bool F = false> CFD_HD void wave_speeds(const Material& m, const double* U, double& su, double& sv, Model<EULER>*)
 {
+    if constexpr (F) {
+        const double r = ddiv<F>(1.0, U[0]);
+        const double w = __builtin_fma(-0.5 * (U[1] * U[1] + U[2] * U[2]), r, U[3]);
+        const double c = sqrt_ratio<F>(__builtin_fma(w, m.wa, m.wb), U[0]);
+        su = __builtin_fma(fabs(U[1]), r, c);
+        sv = __builtin_fma(fabs(U[2]), r, c);
+        return;
+    }
     double rho = U[0];
     double u = ddiv<F>(U[1], rho);
     double v = ddiv<F>(U[2], rho);

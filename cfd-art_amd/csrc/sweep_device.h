@@ -70,12 +70,7 @@ __device__ __forceinline__ void side_from_left(const Side<M>& s, Side<M>& o)
 // are uniform too, so a row's address is formed once in scalar registers.
 __device__ __forceinline__ int wave_index() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
 
-// Row pointers in the global address space (the memory instructions then take a scalar base
-// address and a 32-bit per-lane offset).
-typedef __attribute__((address_space(1))) const double* grow_c;
-typedef __attribute__((address_space(1))) double* grow;
-
-// A wavefront-uniform row pointer kept in scalar registers (the loop optimiser would otherwise
+// A wavefront-uniform row address kept in scalar registers (the loop optimiser would otherwise
 // carry per-lane 64-bit copies of each row pointer across the row loop).
 __device__ __forceinline__ unsigned long long uniform_addr(const void* p)
 {
@@ -84,17 +79,23 @@ __device__ __forceinline__ unsigned long long uniform_addr(const void* p)
     const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
     return ((unsigned long long)hi << 32) | lo;
 }
-__device__ __forceinline__ grow_c uniform_row(const double* p) { return (grow_c)uniform_addr(p); }
-__device__ __forceinline__ grow uniform_row(double* p) { return (grow)uniform_addr(p); }
 
-// Element x (0 <= x < 2^28) of a uniform row: the lane adds a 32-bit byte offset.
-__device__ __forceinline__ grow_c row_elem(grow_c row, int x)
+// A wavefront-uniform row of n doubles as a buffer resource in scalar registers: loads and
+// stores through it take the lane's 32-bit byte offset and need no per-lane address arithmetic.
+// (An offset at or past 8n bytes would read zero and drop the write; callers' columns are in
+// [0, n).) The descriptor's data format is 32-bit, as untyped buffer accesses require on CDNA.
+typedef unsigned int u32x2 __attribute__((__vector_size__(8)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const double* row, int n)
 {
-    return (grow_c)((__attribute__((address_space(1))) const char*)row + (unsigned)x * 8u);
+    return __builtin_amdgcn_make_buffer_rsrc((void*)uniform_addr(row), 0, n * 8, 0x00020000);
 }
-__device__ __forceinline__ grow row_elem(grow row, int x)
+__device__ __forceinline__ double row_load(__amdgpu_buffer_rsrc_t r, int x)
 {
-    return (grow)((__attribute__((address_space(1))) char*)row + (unsigned)x * 8u);
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, x * 8, 0, 0));
+}
+__device__ __forceinline__ void row_store(__amdgpu_buffer_rsrc_t r, int x, double v)
+{
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, x * 8, 0, 0);
 }
 
 // New states are written with ordinary stores, which leave them in the memory-side cache (256 MB
@@ -102,7 +103,6 @@ __device__ __forceinline__ grow row_elem(grow row, int x)
 // each sweep then read its input from HBM, and the memory traffic of a 2048^2 Euler x sweep alone
 // took 56 instead of 44 us.
 __device__ __forceinline__ void store_state(double v, double* p) { *p = v; }
-__device__ __forceinline__ void store_state(double v, grow p) { *p = v; }
 
 // Workgroups go to the 8 XCDs (each with its own L2) round-robin in order of linear id.
 // xcd_order maps the linear id b of n workgroups to a tile order in which each XCD receives
@@ -139,6 +139,13 @@ struct Tally {
     int slope = 0, evolve = 0, repair = 0, bad = 0;
     __device__ static void count(int& n, bool p) { n += __builtin_popcountll(__builtin_amdgcn_ballot_w64(p)); }
     __device__ static void count(int& n, unsigned long long lanes) { n += __builtin_popcountll(lanes); }
+    // running maxima of a final state's speeds: the fast arithmetic takes v_max_f64, which agrees
+    // with smax here (speeds are +0 or more, and a NaN speed keeps the old maximum in both)
+    template <bool F> __device__ void speeds(double a, double b)
+    {
+        su = F ? fmax(su, a) : smax(su, a);
+        sv = F ? fmax(sv, b) : smax(sv, b);
+    }
     // Wavefront-wide reduction of the maxima, then lane 0 issues the atomics into a slot of the control block
     // (slot = the workgroup's number times its wavefront count plus the wavefront's). Every lane
     // of the wavefront must call it. There is no workgroup barrier: a barrier's release fence
@@ -318,12 +325,11 @@ __device__ __forceinline__ void x_line(const GridView& g, double s, double r, in
         // y is wavefront-uniform: each component's row address is a scalar
 #pragma unroll
         for (int k = 0; k < NC; ++k)
-            store_state(Un[k], row_elem(uniform_row(g.nxt + k * g.plane + (long long)y * g.pitch), x));
+            row_store(row_rsrc(g.nxt + k * g.plane + (long long)y * g.pitch, g.pitch), x, Un[k]);
         if (LAST) {
             double su, sv;
             wave_speeds<F>(g.mat, Un, su, sv, (Model<M>*)nullptr);
-            tl.su = smax(tl.su, su);
-            tl.sv = smax(tl.sv, sv);
+            tl.speeds<F>(su, sv);
         }
     }
 }
@@ -334,7 +340,7 @@ __device__ __forceinline__ void x_load(const GridView& g, int tx, int y, double*
 {
     const int xs = mirror(tx * XTILE - 2 + (int)(threadIdx.x & 63), g.nx);
 #pragma unroll
-    for (int k = 0; k < Model<M>::NC; ++k) U[k] = *row_elem(uniform_row(g.cur + k * g.plane + (long long)y * g.pitch), xs);
+    for (int k = 0; k < Model<M>::NC; ++k) U[k] = row_load(row_rsrc(g.cur + k * g.plane + (long long)y * g.pitch, g.pitch), xs);
 }
 
 // One x-sweep tile: the calling wavefront covers columns [tx*60-2, tx*60+62) and rows
@@ -447,8 +453,7 @@ __device__ __forceinline__ void x_tile_godunov(const GridView& g, double dt, int
             if (LAST) {
                 double su, sv;
                 wave_speeds<F>(g.mat, Un, su, sv, (Model<M>*)nullptr);
-                tl.su = smax(tl.su, su);
-                tl.sv = smax(tl.sv, sv);
+                tl.speeds<F>(su, sv);
             }
         }
     }
@@ -557,8 +562,7 @@ __device__ __forceinline__ void y_tile(const GridView& g, double dt, int bx, int
                 if (LAST) {
                     double su, sv;
                     wave_speeds<F>(g.mat, Un, su, sv, (Model<M>*)nullptr);
-                    tl.su = smax(tl.su, su);
-                    tl.sv = smax(tl.sv, sv);
+                    tl.speeds<F>(su, sv);
                 }
             }
         }
@@ -588,15 +592,24 @@ __device__ __forceinline__ void y_tile_t(const GridView& g, double dt, int x0, i
     static_assert(256 % COLS == 0 && COLS % 4 == 0, "a 256-thread tile load covers whole rows; 4 waves share the columns");
     const int tid = threadIdx.x;
     {
+        // every load of the tile is issued before the first LDS write (one memory latency per
+        // tile rather than one per pass)
+        constexpr int PASSES = 64 * COLS / 256;
         const int col = tid % COLS, rsub = tid / COLS;
         const int xs = min(x0 + col, g.nx - 1);
+        double v[PASSES][NC];
 #pragma unroll
-        for (int pass = 0; pass < 64 * COLS / 256; ++pass) {
-            const int r = pass * (256 / COLS) + rsub;
-            const int ys = mirror(y0 - 2 + r, g.ny);
+        for (int pass = 0; pass < PASSES; ++pass) {
+            const int ys = mirror(y0 - 2 + pass * (256 / COLS) + rsub, g.ny);
             const double* b = g.cur + (long long)ys * g.pitch + xs;
 #pragma unroll
-            for (int k = 0; k < NC; ++k) sU[(k * 64 + r) * PAD + col] = b[k * g.plane];
+            for (int k = 0; k < NC; ++k) v[pass][k] = b[k * g.plane];
+        }
+#pragma unroll
+        for (int pass = 0; pass < PASSES; ++pass) {
+            const int r = pass * (256 / COLS) + rsub;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) sU[(k * 64 + r) * PAD + col] = v[pass][k];
         }
     }
     __syncthreads();
@@ -643,8 +656,7 @@ __device__ __forceinline__ void y_tile_t(const GridView& g, double dt, int x0, i
             if (LAST) {
                 double su, sv;
                 wave_speeds<F>(g.mat, Un, su, sv, (Model<M>*)nullptr);
-                tl.su = smax(tl.su, su);
-                tl.sv = smax(tl.sv, sv);
+                tl.speeds<F>(su, sv);
             }
         }
         // the column is this wavefront's alone: its results replace it
@@ -720,8 +732,7 @@ __device__ __forceinline__ void y_tile_godunov(const GridView& g, double dt, int
             if (LAST) {
                 double su, sv;
                 wave_speeds<F>(g.mat, Un, su, sv, (Model<M>*)nullptr);
-                tl.su = smax(tl.su, su);
-                tl.sv = smax(tl.sv, sv);
+                tl.speeds<F>(su, sv);
             }
         }
 #pragma unroll

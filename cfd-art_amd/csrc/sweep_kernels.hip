@@ -118,8 +118,8 @@ __global__ __launch_bounds__(256) void reduce_dt_kernel(GridView g, double* part
         for (int k = 0; k < NC; ++k) U[k] = b[k * g.plane];
         double a, c;
         wave_speeds<F>(g.mat, U, a, c, (Model<M>*)nullptr);
-        su = smax(su, a);
-        sv = smax(sv, c);
+        su = F ? fmax(su, a) : smax(su, a);
+        sv = F ? fmax(sv, c) : smax(sv, c);
         m1 += U[0];
         if (M == ALLAIRE) m2 += U[1];
     }
