@@ -124,11 +124,31 @@ __device__ __forceinline__ int mirror(int i, int n)
     return r < 0 ? 0 : (r >= n ? n - 1 : r);
 }
 
+// A DPP move of a double within the wavefront (both 32-bit halves); lanes of rows outside
+// ROW_MASK keep their own value.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ double dpp_move(double v)
+{
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(v), __double2hiint(v), CTRL, ROW_MASK, 0xf, false);
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(v), __double2loint(v), CTRL, ROW_MASK, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Maximum over the wavefront, received by every lane: pairs, quads (quad_perm), half-rows and
+// rows (mirrors), then rows 0+1 and 2+3 and all four (row_bcast:15 and :31 into the rows of
+// row mask 0xA and 0xC) leave it in row 3, whose lane 63 is read into scalar registers. Every
+// lane must be active. The inputs here are never NaN and never below +0, so v_max_f64 gives the
+// maximum a serial smax fold gives, in any order.
 __device__ __forceinline__ double wave_max(double v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = smax(v, __shfl_xor(v, o));
-    return v;
+    v = fmax(v, dpp_move<0xB1>(v));        // quad_perm [1,0,3,2]
+    v = fmax(v, dpp_move<0x4E>(v));        // quad_perm [2,3,0,1]
+    v = fmax(v, dpp_move<0x141>(v));       // row_half_mirror
+    v = fmax(v, dpp_move<0x140>(v));       // row_mirror
+    v = fmax(v, dpp_move<0x142, 0xA>(v));  // row_bcast:15
+    v = fmax(v, dpp_move<0x143, 0xC>(v));  // row_bcast:31
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 63),
+                            __builtin_amdgcn_readlane(__double2loint(v), 63));
 }
 
 struct Tally {
