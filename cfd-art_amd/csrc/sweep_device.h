@@ -792,14 +792,19 @@ __device__ __forceinline__ void fold_maxima_wave(Ctrl* c, double& mu, double& mv
 }
 
 // dt from the folded maxima (E/sourcecode/euler_timestep.cpp:35-43).
+__device__ __forceinline__ double dt_from_values(double cfl, double T, double mu, double mv, int n, double t, double dx,
+                                                 double dy)
+{
+    double CFL = cfl;
+    if (n < 5) CFL = smin(CFL, 0.2);
+    double dt = CFL * smin(dx / mu, dy / mv);
+    if (t + dt > T) dt = T - t;
+    return dt;
+}
 __device__ __forceinline__ double dt_from_maxima(const Ctrl* c, double mu, double mv, int n, double t, double dx,
                                                  double dy)
 {
-    double CFL = c->cfl;
-    if (n < 5) CFL = smin(CFL, 0.2);
-    double dt = CFL * smin(dx / mu, dy / mv);
-    if (t + dt > c->T) dt = c->T - t;
-    return dt;
+    return dt_from_values(c->cfl, c->T, mu, mv, n, t, dx, dy);
 }
 
 }  // namespace cfd

@@ -204,17 +204,27 @@ __global__ __launch_bounds__(64) void init_dt_kernel(Ctrl* c, double dx, double 
 // there), as it does for a grid in a batch.
 __global__ __launch_bounds__(64) void finalize_step_kernel(Ctrl* c, double dx, double dy)
 {
-    if (!c->active) return;  // the same value in every lane
-    double mu, mv;
-    fold_maxima_wave(c, mu, mv);  // the whole wavefront
-    if (threadIdx.x != 0) return;
-    double t = c->t + c->dt;
-    int n = c->n + 1;
-    if (fabs(t - c->T) < 1e-12) t = c->T;
+    // everything the step's end reads is loaded at once (one memory latency rather than a chain):
+    // lane i's slot maxima and the control scalars
+    const int lane = threadIdx.x & 63;
+    const unsigned long long su = c->maxu[lane], sv = c->maxv[lane];
+    const int active = c->active, n0 = c->n, error = c->error;
+    const double t0 = c->t, dt0 = c->dt, T = c->T, cfl = c->cfl;
+    if (!active) return;  // the same value in every lane
+    c->maxu[lane] = 0;
+    c->maxv[lane] = 0;
+    // the slots hold non-negative doubles (or zero), whose bit patterns order as their values
+    const double mu = wave_max(__longlong_as_double((long long)su));
+    const double mv = wave_max(__longlong_as_double((long long)sv));
+    if (lane != 0) return;
+    double t = t0 + dt0;
+    const int n = n0 + 1;
+    if (fabs(t - T) < 1e-12) t = T;
     c->t = t;
     c->n = n;
-    c->active = t < c->T && !c->error;
-    if (c->active) c->dt = dt_from_maxima(c, mu, mv, n, t, dx, dy);
+    const bool go = t < T && !error;
+    c->active = go;
+    if (go) c->dt = dt_from_values(cfl, T, mu, mv, n, t, dx, dy);
 }
 
 __global__ void set_run_kernel(Ctrl* c, double T, double cfl)
