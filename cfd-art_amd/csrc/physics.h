@@ -878,8 +878,22 @@ template <bool F = false> CFD_HD void wave_speeds(const Material& m, const doubl
     su = fabs(u) + c;
     sv = fabs(v) + c;
 }
+// The fast arithmetic forms the Allaire speeds from the reciprocals of rho and of eps(z):
+// rho e = E - (mu^2 + mv^2)/(2 rho), p = rho e / eps - k and rho c^2 = k + p (1/eps + 1).
 template <bool F = false> CFD_HD void wave_speeds(const Material& m, const double* U, double& su, double& sv, Model<ALLAIRE>*)
 {
+    if constexpr (F) {
+        const double rho = U[0] + U[1];
+        const double r = ddiv<F>(1.0, rho);
+        const double w = __builtin_fma(-0.5 * (U[2] * U[2] + U[3] * U[3]), r, U[4]);
+        const double reps = ddiv<F>(1.0, __builtin_fma(U[5], m.fe12, m.fe2));
+        const double k = __builtin_fma(U[5], m.fk12, m.fk2) * reps;
+        const double p = __builtin_fma(w, reps, -k);
+        const double c = sqrt_ratio<F>(k + p * (reps + 1.0), rho);
+        su = __builtin_fma(fabs(U[2]), r, c);
+        sv = __builtin_fma(fabs(U[3]), r, c);
+        return;
+    }
     double rho = U[0] + U[1];
     double u = ddiv<F>(U[2], rho);
     double v = ddiv<F>(U[3], rho);
