@@ -424,11 +424,11 @@ int cfdart_compute_dt(cfdart_grid* g, int n, double t, double T, double cfl, dou
     HIPCHK(hipMemcpyAsync(&c, g->ctrl, sizeof(c), hipMemcpyDeviceToHost, g->st));
     HIPCHK(hipMemcpyAsync(part.data(), g->partials, sizeof(double) * part.size(), hipMemcpyDeviceToHost, g->st));
     HIPCHK(hipStreamSynchronize(g->st));
-    HIPCHK(hipMemsetAsync(&g->ctrl->maxu[0], 0, 2 * cfd::MAX_SLOTS * sizeof(unsigned long long), g->st));
+    HIPCHK(hipMemsetAsync(&g->ctrl->slots[0][0], 0, sizeof(g->ctrl->slots), g->st));
     unsigned long long mu = 0, mv = 0;
     for (int i = 0; i < cfd::MAX_SLOTS; ++i) {
-        mu = c.maxu[i] > mu ? c.maxu[i] : mu;
-        mv = c.maxv[i] > mv ? c.maxv[i] : mv;
+        mu = c.maxu(i) > mu ? c.maxu(i) : mu;
+        mv = c.maxv(i) > mv ? c.maxv(i) : mv;
     }
     double maxu, maxv;
     std::memcpy(&maxu, &mu, sizeof(double));

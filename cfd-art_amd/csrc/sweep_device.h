@@ -179,8 +179,8 @@ struct Tally {
         if ((threadIdx.x & 63) == 0) {
             const int sl = (slot * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6)) & (MAX_SLOTS - 1);
             if (last) {
-                atomicMax(&c->maxu[sl], (unsigned long long)__double_as_longlong(mu));
-                atomicMax(&c->maxv[sl], (unsigned long long)__double_as_longlong(mv));
+                atomicMax(&c->maxu(sl), (unsigned long long)__double_as_longlong(mu));
+                atomicMax(&c->maxv(sl), (unsigned long long)__double_as_longlong(mv));
             }
             if (a) atomicAdd(&c->cnt[0], (unsigned long long)a);
             if (b) atomicAdd(&c->cnt[1], (unsigned long long)b);
@@ -767,10 +767,10 @@ __device__ __forceinline__ void fold_maxima(Ctrl* c, double& mu, double& mv)
 {
     unsigned long long a = 0, b = 0;
     for (int i = 0; i < MAX_SLOTS; ++i) {
-        a = c->maxu[i] > a ? c->maxu[i] : a;
-        b = c->maxv[i] > b ? c->maxv[i] : b;
-        c->maxu[i] = 0;
-        c->maxv[i] = 0;
+        a = c->maxu(i) > a ? c->maxu(i) : a;
+        b = c->maxv(i) > b ? c->maxv(i) : b;
+        c->maxu(i) = 0;
+        c->maxv(i) = 0;
     }
     mu = __longlong_as_double((long long)a);
     mv = __longlong_as_double((long long)b);
@@ -783,10 +783,10 @@ __device__ __forceinline__ void fold_maxima_wave(Ctrl* c, double& mu, double& mv
 {
     static_assert(MAX_SLOTS == 64, "one slot per lane");
     const int lane = threadIdx.x & 63;
-    const double a = __longlong_as_double((long long)c->maxu[lane]);
-    const double b = __longlong_as_double((long long)c->maxv[lane]);
-    c->maxu[lane] = 0;
-    c->maxv[lane] = 0;
+    const double a = __longlong_as_double((long long)c->maxu(lane));
+    const double b = __longlong_as_double((long long)c->maxv(lane));
+    c->maxu(lane) = 0;
+    c->maxv(lane) = 0;
     mu = wave_max(a);
     mv = wave_max(b);
 }

@@ -15,12 +15,18 @@ constexpr int MAX_SLOTS = 64;  // spread of the wave-speed maxima over addresses
 struct Ctrl {
     double t, dt, T, cfl;
     int n, active;
-    // bit patterns of non-negative doubles, max-reduced by workgroups into slot (id % MAX_SLOTS)
-    unsigned long long maxu[MAX_SLOTS], maxv[MAX_SLOTS];
     unsigned long long cnt[4];      // slope fallbacks, evolve fallbacks, repairs, unphysical
     double mass_partial_unused;
     int error;
     int pad;
+    // bit patterns of non-negative doubles, max-reduced by wavefronts into slot (id % MAX_SLOTS):
+    // slot i's u and v maxima share a 128-byte line of their own, so the atomics of different
+    // slots go to different lines (and memory channels) instead of queueing on a few lines
+    alignas(128) unsigned long long slots[MAX_SLOTS][16];
+    __host__ __device__ unsigned long long& maxu(int i) { return slots[i][0]; }
+    __host__ __device__ unsigned long long& maxv(int i) { return slots[i][1]; }
+    __host__ __device__ const unsigned long long& maxu(int i) const { return slots[i][0]; }
+    __host__ __device__ const unsigned long long& maxv(int i) const { return slots[i][1]; }
 };
 
 // Device view of one grid: SoA planes, interior cells only (the ghost frame is produced on

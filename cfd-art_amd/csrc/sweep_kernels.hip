@@ -207,12 +207,12 @@ __global__ __launch_bounds__(64) void finalize_step_kernel(Ctrl* c, double dx, d
     // everything the step's end reads is loaded at once (one memory latency rather than a chain):
     // lane i's slot maxima and the control scalars
     const int lane = threadIdx.x & 63;
-    const unsigned long long su = c->maxu[lane], sv = c->maxv[lane];
+    const unsigned long long su = c->maxu(lane), sv = c->maxv(lane);
     const int active = c->active, n0 = c->n, error = c->error;
     const double t0 = c->t, dt0 = c->dt, T = c->T, cfl = c->cfl;
     if (!active) return;  // the same value in every lane
-    c->maxu[lane] = 0;
-    c->maxv[lane] = 0;
+    c->maxu(lane) = 0;
+    c->maxv(lane) = 0;
     // the slots hold non-negative doubles (or zero), whose bit patterns order as their values
     const double mu = wave_max(__longlong_as_double((long long)su));
     const double mv = wave_max(__longlong_as_double((long long)sv));

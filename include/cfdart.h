@@ -162,7 +162,7 @@ int cfdart_set_tile(cfdart_grid* g, int y_strip_rows, int x_rows_per_wave);
  * with the default tile shapes, and the final state ends in 'state'. The loop's control block and
  * reduction partials live in a 32 KiB device array of the library, one per GPU and held until the
  * process ends; a grid whose partials would not fit is refused with CFDART_E_ARG (none does: the
- * need is at most 17.25 KiB). Calls that overlap on one GPU share that array and race. Asynchronous
+ * need is at most 24.25 KiB). Calls that overlap on one GPU share that array and race. Asynchronous
  * unless t_out or n_out is non-null (then as cfdart_run, including CFDART_E_UNPHYS). Results are
  * bitwise those of cfdart_run on a handle in the same arithmetic. */
 int cfdart_run_noalloc(const cfdart_desc* d, double* state, double* work, int pitch, int arith, int splitting,
