@@ -180,8 +180,9 @@ def bench_served(args):
     """Served stream: Poisson arrivals (50/s, 70 % Euler, sides log-uniform in [64, 2048], steps
     uniform in [50, 2000], a frame every 50 steps, 200 GB admission budget, at most 32 M live
     cells) replayed in real time through the continuous-batching server, then the same requests
-    one at a time with the plain solver. The window is shorter than the full 120 s so that the run stays within minutes; the
-    offered load (about 31 G cell-updates/s) exceeds what one GPU serves, so requests queue."""
+    one at a time with the plain solver. The default window (10 s) keeps the run within a minute or so;
+    --serve-window 120 replays the configured stream. The offered load (about 31 G cell-updates/s)
+    exceeds what one GPU serves, so requests queue. The server prints a progress line every 30 s."""
     import cfdart
     from cfdart import serve
 
@@ -198,7 +199,8 @@ def bench_served(args):
     def opts(cap):
         return serve.opts(mem_budget=200 * 10**9, capacity=8192, chunk_steps=10, frame_every=50,
                           frame_host_bytes=2 << 30, realtime=True, tile=tile, plain_tile=plain, live_cells=cap,
-                          arith=cfdart.ARITH_EXACT if args.arith == "exact" else cfdart.ARITH_FAST)
+                          arith=cfdart.ARITH_EXACT if args.arith == "exact" else cfdart.ARITH_FAST,
+                          progress_every=30.0)
 
     o = opts(live_cap)
     rb, sb = serve.serve_batched(reqs, o)

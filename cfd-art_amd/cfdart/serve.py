@@ -17,7 +17,7 @@ class ServeOpts(C.Structure):
                 ("frame_every", C.c_int), ("frame_host_bytes", C.c_size_t), ("realtime", C.c_int),
                 ("y_strip_rows", C.c_int * 2), ("x_rows_per_wave", C.c_int * 2),
                 ("plain_y_strip_rows", C.c_int * 2), ("plain_x_rows_per_wave", C.c_int * 2),
-                ("live_cells", C.c_longlong), ("arith", C.c_int)]
+                ("live_cells", C.c_longlong), ("arith", C.c_int), ("progress_every", C.c_double)]
 
 
 class RequestResult(C.Structure):
@@ -85,10 +85,11 @@ def setup(req):
 
 
 def opts(mem_budget=200 * 10**9, capacity=4096, chunk_steps=10, frame_every=50, frame_host_bytes=1 << 30,
-         realtime=True, tile=None, plain_tile=None, live_cells=0, arith=0):
+         realtime=True, tile=None, plain_tile=None, live_cells=0, arith=0, progress_every=0.0):
     o = ServeOpts()
     o.live_cells = int(live_cells)
     o.arith = int(arith)
+    o.progress_every = float(progress_every)
     o.mem_budget, o.capacity, o.chunk_steps, o.frame_every = int(mem_budget), capacity, chunk_steps, frame_every
     o.frame_host_bytes, o.realtime = int(frame_host_bytes), 1 if realtime else 0
     for m in (0, 1):

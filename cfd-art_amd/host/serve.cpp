@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <deque>
 #include <random>
@@ -179,8 +180,15 @@ int cfdart_serve_batched(const cfdart_request* reqs, int n, const cfdart_serve_o
         cfdart_batch_destroy(b);
         return code;
     };
+    double reported = 0.0;
     while (done < n) {
         const double now = seconds_since(t0);
+        if (o->progress_every > 0 && now - reported >= o->progress_every) {
+            std::fprintf(stderr, "serve batched: %.1f s, %d/%d requests finished, %d live, %zu waiting\n", now, done, n,
+                         live, waiting.size() + (size_t)(n - next));
+            std::fflush(stderr);
+            reported = now;
+        }
         while (next < n && (!o->realtime || reqs[next].arrival <= now)) waiting.push_back(next++);
         // admission, first come first served: stop at the first request that does not fit
         while (!waiting.empty()) {
@@ -284,7 +292,14 @@ int cfdart_serve_one_at_a_time(const cfdart_request* reqs, int n, const cfdart_s
     std::memset(out, 0, sizeof(cfdart_request_result) * (size_t)n);
     FrameRing ring(o->frame_every > 0 ? o->frame_host_bytes : 0);
     double clock = 0.0;  // the schedule a one-at-a-time server would follow
+    const Clock::time_point start = Clock::now();
+    double reported = 0.0;
     for (int r = 0; r < n; ++r) {
+        if (o->progress_every > 0 && seconds_since(start) - reported >= o->progress_every) {
+            reported = seconds_since(start);
+            std::fprintf(stderr, "serve one at a time: %.1f s, %d/%d requests, schedule at %.1f s\n", reported, r, n, clock);
+            std::fflush(stderr);
+        }
         cfdart_desc d;
         cfdart_disc_params dp;
         cfdart_bubble_params bp;

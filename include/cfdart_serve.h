@@ -69,6 +69,8 @@ typedef struct cfdart_serve_opts {
                                 (enough to fill the GPU; the rest wait first come first served,
                                 which shortens each admitted request's latency); 0 = no limit */
     int arith;               /* CFDART_ARITH_EXACT (0) or CFDART_ARITH_FAST, batched and one at a time */
+    double progress_every;   /* > 0: a progress line on stderr at most this many seconds apart
+                                (requests finished, live, waiting); 0 = silent */
 } cfdart_serve_opts;
 
 typedef struct cfdart_request_result {
