@@ -3,8 +3,12 @@ compiler's IEEE sequences without their operand scaling) return exactly the IEEE
 for bit, against numpy's correctly rounded a / b and sqrt and against the compiler's own a / b
 and sqrt on the device. The operand classes cover the range the solver's states span and far
 beyond it (magnitudes 1e-280..1e280, quotients up to 1e310), exact quotients, zeros, infinities
-and NaNs. Subnormal operands and numerators below 2^-969 in magnitude, where the omitted scaling
-acts, are outside the solver's range and are not claimed."""
+and NaNs, and each edge of the range in which the omitted scaling is inactive: numerators down
+to 2^-969, divisors up to 2^1022, quotients down to 2^-1022, exponent gaps up to 767, roots of
+numbers down to 2^-767. Just past those edges (subnormal operands or quotients, numerators
+below 2^-969, divisors of 2^1022 and more, gaps of 768 and more, roots below 2^-767) the
+scaling acts and the results can differ; the last test shows that they do for subnormal
+operands and for roots below 2^-767 (its other classes differed nowhere in these samples)."""
 import ctypes as C
 import os
 
@@ -67,3 +71,66 @@ def test_exact_quotients_and_special_values(gpu):
         q, s = A / B, np.sqrt(A)
     assert same(out[:, 0], q).all()
     assert same(out[:, 2], s).all()
+
+
+def mag(rng, n, lo, hi):
+    """n values 2^e m with e uniform in [lo, hi), m uniform in [1, 2), random signs."""
+    return np.ldexp(rng.uniform(1.0, 2.0, n), rng.integers(lo, hi, n)) * rng.choice([-1.0, 1.0], n)
+
+
+def subnormal(rng, n):
+    return rng.integers(1, 1 << 52, n).astype(np.uint64).view(np.float64) * rng.choice([-1.0, 1.0], n)
+
+
+def test_edges_of_the_unscaled_range(gpu):
+    """Operands at the edges of the range in which the compiler's sequences scale nothing."""
+    rng = np.random.default_rng(11)
+    n = 1 << 16
+    cases = {
+        "numerators 2^-969..2^-940": (mag(rng, n, -969, -940), mag(rng, n, -40, 40)),
+        "divisors 2^1000..2^1022": (mag(rng, n, 990, 1023), mag(rng, n, 1000, 1022)),
+        "quotients 2^-1021..2^-899": (mag(rng, n, -900, -800), mag(rng, n, 100, 121)),
+        "exponent gaps 568..767": (mag(rng, n, 500, 501), mag(rng, n, -267, -67)),
+    }
+    for name, (a, b) in cases.items():
+        out = run(a, b)
+        with np.errstate(all="ignore"):
+            q = a / b
+        print(f"case {name}: {a.size} pairs", flush=True)
+        assert np.isfinite(q).all() and (np.abs(q) >= 2.0 ** -1022).all(), name
+        assert same(out[:, 1], q).all(), name
+        assert same(out[:, 0], q).all(), (name, int((~same(out[:, 0], q)).sum()))
+    x = mag(rng, n, -767, -700)
+    out = run(x, np.ones(n))
+    assert same(out[:, 3], np.sqrt(x)).all() and same(out[:, 2], np.sqrt(x)).all()
+
+
+def test_past_the_edges_the_scaling_acts(gpu):
+    """Operands the compiler's sequences rescale: those sequences stay IEEE, and the solver's
+    unscaled ones differ on some of them (the range the exact arithmetic excludes). The counts
+    per class are printed; subnormal operands and roots below 2^-767 are where they differ."""
+    rng = np.random.default_rng(12)
+    n = 1 << 16
+    cases = {
+        "subnormal numerator": (subnormal(rng, n), mag(rng, n, -60, 60)),
+        "subnormal divisor": (mag(rng, n, -60, 60), subnormal(rng, n)),
+        "numerator below 2^-969": (mag(rng, n, -1000, -969), mag(rng, n, -40, 40)),
+        "subnormal quotient": (mag(rng, n, -1000, -960), mag(rng, n, 30, 120)),
+        "divisor from 2^1022": (mag(rng, n, -40, 40), mag(rng, n, 1022, 1024)),
+        "exponent gap from 768": (mag(rng, n, 500, 501), mag(rng, n, -300, -267)),
+    }
+    differ = 0
+    for name, (a, b) in cases.items():
+        out = run(a, b)
+        with np.errstate(all="ignore"):
+            q = a / b
+        d = int((~same(out[:, 0], q)).sum())
+        print(f"case {name}: {a.size} pairs, {d} unscaled quotients differ from IEEE", flush=True)
+        assert same(out[:, 1], q).all(), name
+        differ += d
+    x = np.concatenate([mag(rng, n, -1000, -767), subnormal(rng, n)])
+    out = run(x, np.ones(x.size))
+    ds = int((~same(out[:, 2], np.sqrt(x))).sum())
+    print(f"roots below 2^-767: {x.size} values, {ds} unscaled roots differ from IEEE", flush=True)
+    assert same(out[:, 3], np.sqrt(x)).all()
+    assert differ > 0 and ds > 0
