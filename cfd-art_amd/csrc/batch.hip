@@ -165,13 +165,15 @@ int rebuild_tiles(cfdart_batch* b)
     }
     b->ntiles = 0;
     BatchQueue lengths{};
+    std::vector<Tile> lists[2][2];  // the host side of the copies below, kept until they finish
     for (int m = 0; m < 2; ++m)
         for (int ax = 0; ax < 2; ++ax) {
             auto& l = v[m][ax];
             std::stable_sort(l.begin(), l.end(), [](const std::pair<double, Tile>& x, const std::pair<double, Tile>& y) {
                 return x.first > y.first;
             });
-            std::vector<Tile> t(l.size());
+            std::vector<Tile>& t = lists[m][ax];
+            t.resize(l.size());
             for (size_t i = 0; i < l.size(); ++i) t[i] = l[i].second;
             if (t.size() > b->tile_cap[m][ax]) {
                 BCHK(hipStreamSynchronize(b->st));
@@ -190,6 +192,7 @@ int rebuild_tiles(cfdart_batch* b)
         }
     hipLaunchKernelGGL(cfd::batch_set_lengths_kernel, dim3(1), dim3(64), 0, b->st, b->d_q, lengths);
     BCHK(hipGetLastError());
+    BCHK(hipStreamSynchronize(b->st));  // the host lists go out of scope
     return 0;
 }
 
@@ -425,7 +428,11 @@ int admit(cfdart_batch* b, const cfdart_desc* d, const double* host, const cfdar
             return bfail(rc, err);
         }
     }
-    BCHK(hipMemcpyAsync(b->d_grids + slot, &g, sizeof(BatchGrid), hipMemcpyHostToDevice, b->st));
+    hipError_t e = hipMemcpyAsync(b->d_grids + slot, &g, sizeof(BatchGrid), hipMemcpyHostToDevice, b->st);
+    if (e != hipSuccess) {
+        release();
+        return bfail(CFDART_E_HIP, std::string("grid table upload: ") + hipGetErrorString(e));
+    }
     // first dt: compute_dt's reduction on the new state into the grid's control block
     cfd::GridView v;
     v.cur = g.buf[0];
@@ -442,14 +449,17 @@ int admit(cfdart_batch* b, const cfdart_desc* d, const double* host, const cfdar
     v.dy = g.dy;
     v.dx_half_col = g.dx_half_col;
     v.ctrl = &(b->d_grids + slot)->ctrl;
-    hipError_t e = cfd::launch_reduce_dt(v, g.model, b->d_partials, b->st);
+    e = cfd::launch_reduce_dt(v, g.model, b->d_partials, b->st);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(cfd::batch_admit_kernel, dim3(1), dim3(64), 0, b->st, b->d_grids + slot,
                            b->d_status + 2 * slot);
         e = hipGetLastError();
     }
-    if (e != hipSuccess) return bfail(CFDART_E_HIP, std::string("admission kernels: ") + hipGetErrorString(e));
-    BCHK(hipStreamSynchronize(b->st));  // the staging vector goes out of scope
+    if (e == hipSuccess) e = hipStreamSynchronize(b->st);  // the staging vector and g go out of scope
+    if (e != hipSuccess) {
+        release();
+        return bfail(CFDART_E_HIP, std::string("admission kernels: ") + hipGetErrorString(e));
+    }
     b->h_grids[slot] = g;
     b->live[slot] = 1;
     b->bytes[slot] = need;

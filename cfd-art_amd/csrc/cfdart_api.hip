@@ -478,7 +478,11 @@ int cfdart_run_mass(cfdart_grid* g, int splitting, int max_steps, double T, doub
     double* d_hist = nullptr;
     if (mass_hist && max_steps > 0) {
         HIPCHK(hipMalloc(&d_hist, sizeof(double) * 3 * (size_t)max_steps));
-        HIPCHK(hipMemsetAsync(d_hist, 0, sizeof(double) * 3 * (size_t)max_steps, g->st));
+        e = hipMemsetAsync(d_hist, 0, sizeof(double) * 3 * (size_t)max_steps, g->st);
+        if (e != hipSuccess) {
+            (void)hipFree(d_hist);
+            return fail(CFDART_E_HIP, std::string("clearing the mass history: ") + hipGetErrorString(e));
+        }
     }
     auto release = [&]() {
         if (d_hist) (void)hipFree(d_hist);

@@ -16,7 +16,6 @@ struct Ctrl {
     double t, dt, T, cfl;
     int n, active;
     unsigned long long cnt[4];      // slope fallbacks, evolve fallbacks, repairs, unphysical
-    double mass_partial_unused;
     int error;
     int pad;
     // bit patterns of non-negative doubles, max-reduced by wavefronts into slot (id % MAX_SLOTS):
