@@ -329,10 +329,10 @@ int cfdart_serve_one_at_a_time(const cfdart_request* reqs, int n, const cfdart_s
         while (!rc && taken < reqs[r].steps) {
             const int k = std::min(every, reqs[r].steps - taken);
             double t;
-            int nn;
+            int nn = taken;
             rc = cfdart_run(g, CFDART_ALTERNATING, k, INFINITY, cfl, &t, &nn);
+            taken = nn;  // also when the run stopped on an unphysical state, as the batch records it
             if (rc) break;
-            taken = nn;
             if (o->frame_every > 0 && taken % o->frame_every == 0) {
                 double* dst = ring.take(frame_bytes(d));
                 if (dst && (rc = cfdart_frame(g, dst)) == 0) {
@@ -342,7 +342,11 @@ int cfdart_serve_one_at_a_time(const cfdart_request* reqs, int n, const cfdart_s
                 }
             }
         }
-        if (!rc) rc = cfdart_checksum(g, &res.checksum);
+        // a grid stopped by an unphysical state still reports its checksum, as the batch does
+        if (!rc || rc == CFDART_E_UNPHYS) {
+            const int c = cfdart_checksum(g, &res.checksum);
+            if (c) rc = c;
+        }
         if (!rc) rc = cfdart_sync(g);
         res.service = seconds_since(t0);
         cfdart_destroy(g);
