@@ -139,21 +139,21 @@ void harvest_profile(cfdart_grid* g)
 int sweep_once(cfdart_grid* g, int axis, bool last, double dt_arg, double factor, int use_ctrl)
 {
     cfd::GridView v = view(g);
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    // profiling: the launch itself stamps a pair of events with the kernel's start and end
+    cfd::Stamps ts;
     if (g->profiling) {
         if (g->ev_used + 2 > 4096) {
             if (hipStreamSynchronize(g->st) != hipSuccess) return fail(CFDART_E_HIP, "sync failed");
             harvest_profile(g);
         }
         size_t i0 = g->ev_used;
-        e0 = next_event(g);
-        e1 = next_event(g);
-        if (!e0 || !e1 || hipEventRecord(e0, g->st) != hipSuccess) return fail(CFDART_E_HIP, "event record failed");
+        ts.start = next_event(g);
+        ts.stop = next_event(g);
+        if (!ts.start || !ts.stop) return fail(CFDART_E_HIP, "event creation failed");
         g->ev_marks.push_back({axis, (int)i0});
     }
-    hipError_t e = cfd::launch_sweep(v, g->d.model, axis, last, dt_arg, factor, use_ctrl, g->yH, g->xrows, g->st);
+    hipError_t e = cfd::launch_sweep(v, g->d.model, axis, last, dt_arg, factor, use_ctrl, g->yH, g->xrows, g->st, ts);
     if (e != hipSuccess) return fail(CFDART_E_HIP, std::string("sweep launch: ") + hipGetErrorString(e));
-    if (g->profiling && hipEventRecord(e1, g->st) != hipSuccess) return fail(CFDART_E_HIP, "event record failed");
     g->cur = 1 - g->cur;  // post_sweep: grid.swap(future_grid)
     return 0;
 }

@@ -672,19 +672,16 @@ __device__ __forceinline__ void y_tile_t(const GridView& g, double dt, int x0, i
         Tally::count(tl.evolve, efb & counted);
         Tally::count(tl.repair, rep);
         Tally::count(tl.bad, bad);
-        if (store) {
-            if (LAST) {
-                double su, sv;
-                wave_speeds<F>(g.mat, Un, su, sv, (Model<M>*)nullptr);
-                tl.speeds<F>(su, sv);
-            }
-        }
         // the column is this wavefront's alone: its results replace it
 #pragma unroll
         for (int k = 0; k < NC; ++k) sU[(k * 64 + lane) * PAD + c] = Un[k];
     }
     __syncthreads();
     {
+        // The last sweep of a step takes each new cell's wave speeds here, from the state it
+        // stores: the same cells (rows 2..61 of the tile inside the grid) and values as in the
+        // line loop above, but with few registers live, so the line loop keeps the occupancy of
+        // a sweep that does not end a step.
         const int col = tid % COLS, rsub = tid / COLS;
         const int x = x0 + col;
         if (x < g.nx) {
@@ -694,8 +691,17 @@ __device__ __forceinline__ void y_tile_t(const GridView& g, double dt, int x0, i
                 const int yy = y0 - 2 + rr;
                 if (rr >= 2 && rr < 62 && yy < g.ny) {
                     double* out = g.nxt + (long long)yy * g.pitch + x;
+                    double Un[NC];
 #pragma unroll
-                    for (int k = 0; k < NC; ++k) store_state(sU[(k * 64 + rr) * PAD + col], out + k * g.plane);
+                    for (int k = 0; k < NC; ++k) {
+                        Un[k] = sU[(k * 64 + rr) * PAD + col];
+                        store_state(Un[k], out + k * g.plane);
+                    }
+                    if (LAST) {
+                        double su, sv;
+                        wave_speeds<F>(g.mat, Un, su, sv, (Model<M>*)nullptr);
+                        tl.speeds<F>(su, sv);
+                    }
                 }
             }
         }

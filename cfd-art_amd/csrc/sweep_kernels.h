@@ -55,14 +55,19 @@ constexpr bool y_transposed(int yH) { return yH == Y_TRANSPOSED || yH == Y_TRANS
 constexpr int yt_cols(int yH) { return yH == Y_TRANSPOSED_NARROW ? 8 : 16; }
 constexpr int BATCH_YT_COLS = 16;  // the batch's y tile: the 16-column transposed tile
 
-// Host-side launchers (sweep_kernels.hip).
+// Host-side launchers (sweep_kernels.hip). A sweep launch given a pair of events records in them
+// the start and end of the kernel's own execution (hipExtLaunchKernelGGL); without (the default)
+// it is a plain launch.
+struct Stamps {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
 hipError_t launch_sweep(const GridView& g, int model, int axis, bool last, double dt_arg,
-                        double dt_factor, int use_ctrl, int yH, int xrows, hipStream_t st);
+                        double dt_factor, int use_ctrl, int yH, int xrows, hipStream_t st, Stamps ts = Stamps());
 hipError_t launch_reduce_dt(const GridView& g, int model, double* partials, hipStream_t st);
 // The fast arithmetic's launchers (sweep_kernels_fast.hip); launch_sweep and launch_reduce_dt
 // forward to them for a grid whose opt.fast is set.
 hipError_t launch_sweep_fast(const GridView& g, int model, int axis, bool last, double dt_arg, double dt_factor,
-                             int use_ctrl, int yH, int xrows, hipStream_t st);
+                             int use_ctrl, int yH, int xrows, hipStream_t st, Stamps ts);
 hipError_t launch_reduce_dt_fast(const GridView& g, int model, double* partials, hipStream_t st);
 hipError_t launch_init_dt(Ctrl* c, double dx, double dy, hipStream_t st);
 // Mass history entry of the current state (t, mass1, mass2 into hist[0..2]) when the time loop

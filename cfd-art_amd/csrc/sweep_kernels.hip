@@ -20,6 +20,7 @@
 // allowed, and that arithmetic's launchers.
 #include "sweep_device.h"
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #ifndef CFD_FAST_TU
@@ -251,43 +252,56 @@ hipError_t launch_set_time(Ctrl* c, double t, int n, hipStream_t st)
 
 #endif  // !CFD_FAST_TU
 
+// A sweep launch on stream st. With a pair of events (stamps.start non-null) the launch goes
+// through hipExtLaunchKernelGGL, which stamps the two events with the dispatch's own start and
+// end times: their elapsed time is the kernel's execution, as a profiler's kernel trace reports
+// it, without the time the stream spends processing separate event records around the launch.
+#define CFD_SWEEP_LAUNCH(K, grid, stamps, st, ...)                                                         \
+    do {                                                                                                 \
+        if ((stamps).start) hipExtLaunchKernelGGL(K, grid, dim3(256), 0, st, (stamps).start, (stamps).stop, 0, __VA_ARGS__); \
+        else hipLaunchKernelGGL(K, grid, dim3(256), 0, st, __VA_ARGS__);                                 \
+    } while (0)
+
 template <int M, bool LAST, bool GOD, bool F>
-static hipError_t launch_x(const GridView& g, double dt_arg, double f, int use_ctrl, int xrows, hipStream_t st)
+static hipError_t launch_x(const GridView& g, double dt_arg, double f, int use_ctrl, int xrows, hipStream_t st,
+                           Stamps ts)
 {
     dim3 grid((g.nx + XTILE - 1) / XTILE, (g.ny + 4 * xrows - 1) / (4 * xrows));
-    hipLaunchKernelGGL((sweep_x_kernel<M, LAST, GOD, F>), grid, dim3(256), 0, st, g, dt_arg, f, use_ctrl, xrows);
+    CFD_SWEEP_LAUNCH((sweep_x_kernel<M, LAST, GOD, F>), grid, ts, st, g, dt_arg, f, use_ctrl, xrows);
     return hipGetLastError();
 }
 template <int M, bool LAST, bool GOD, bool F>
-static hipError_t launch_y(const GridView& g, double dt_arg, double f, int use_ctrl, int H, hipStream_t st)
+static hipError_t launch_y(const GridView& g, double dt_arg, double f, int use_ctrl, int H, hipStream_t st, Stamps ts)
 {
     if (y_transposed(H) && !GOD) {
         const int cols = yt_cols(H);
         dim3 grid((g.nx + cols - 1) / cols, (g.ny + YT_ROWS - 1) / YT_ROWS);
-        if (cols == 16) hipLaunchKernelGGL((sweep_yt_kernel<M, LAST, 16, F>), grid, dim3(256), 0, st, g, dt_arg, f, use_ctrl);
-        else hipLaunchKernelGGL((sweep_yt_kernel<M, LAST, 8, F>), grid, dim3(256), 0, st, g, dt_arg, f, use_ctrl);
+        if (cols == 16) CFD_SWEEP_LAUNCH((sweep_yt_kernel<M, LAST, 16, F>), grid, ts, st, g, dt_arg, f, use_ctrl);
+        else CFD_SWEEP_LAUNCH((sweep_yt_kernel<M, LAST, 8, F>), grid, ts, st, g, dt_arg, f, use_ctrl);
         return hipGetLastError();
     }
     if (H <= 0) H = 8;  // the first-order scheme has no transposed tile: 8-row strips (its fastest at 2048^2)
+    static_assert(YBLOCK == 256, "strip tiles launch 256-thread workgroups");
     dim3 grid((g.nx + YBLOCK - 1) / YBLOCK, (g.ny + H - 1) / H);
-    hipLaunchKernelGGL((sweep_y_kernel<M, LAST, GOD, F>), grid, dim3(YBLOCK), 0, st, g, dt_arg, f, use_ctrl, H);
+    CFD_SWEEP_LAUNCH((sweep_y_kernel<M, LAST, GOD, F>), grid, ts, st, g, dt_arg, f, use_ctrl, H);
     return hipGetLastError();
 }
+#undef CFD_SWEEP_LAUNCH
 template <int M, bool GOD, bool F>
 static hipError_t launch_model(const GridView& g, int axis, bool last, double dt_arg, double dt_factor, int use_ctrl,
-                               int yH, int xrows, hipStream_t st)
+                               int yH, int xrows, hipStream_t st, Stamps ts)
 {
-    if (axis == 0) return last ? launch_x<M, true, GOD, F>(g, dt_arg, dt_factor, use_ctrl, xrows, st)
-                               : launch_x<M, false, GOD, F>(g, dt_arg, dt_factor, use_ctrl, xrows, st);
-    return last ? launch_y<M, true, GOD, F>(g, dt_arg, dt_factor, use_ctrl, yH, st)
-                : launch_y<M, false, GOD, F>(g, dt_arg, dt_factor, use_ctrl, yH, st);
+    if (axis == 0) return last ? launch_x<M, true, GOD, F>(g, dt_arg, dt_factor, use_ctrl, xrows, st, ts)
+                               : launch_x<M, false, GOD, F>(g, dt_arg, dt_factor, use_ctrl, xrows, st, ts);
+    return last ? launch_y<M, true, GOD, F>(g, dt_arg, dt_factor, use_ctrl, yH, st, ts)
+                : launch_y<M, false, GOD, F>(g, dt_arg, dt_factor, use_ctrl, yH, st, ts);
 }
 
 template <bool F>
 static hipError_t launch_sweep_t(const GridView& g, int model, int axis, bool last, double dt_arg, double dt_factor,
-                                 int use_ctrl, int yH, int xrows, hipStream_t st)
+                                 int use_ctrl, int yH, int xrows, hipStream_t st, Stamps ts)
 {
-#define CFD_LAUNCH(M, GOD) launch_model<M, GOD, F>(g, axis, last, dt_arg, dt_factor, use_ctrl, yH, xrows, st)
+#define CFD_LAUNCH(M, GOD) launch_model<M, GOD, F>(g, axis, last, dt_arg, dt_factor, use_ctrl, yH, xrows, st, ts)
     if (model == EULER) return g.opt.godunov ? CFD_LAUNCH(EULER, true) : CFD_LAUNCH(EULER, false);
     return g.opt.godunov ? CFD_LAUNCH(ALLAIRE, true) : CFD_LAUNCH(ALLAIRE, false);
 #undef CFD_LAUNCH
@@ -304,9 +318,9 @@ static hipError_t launch_reduce_dt_t(const GridView& g, int model, double* parti
 
 #if CFD_FAST_TU
 hipError_t launch_sweep_fast(const GridView& g, int model, int axis, bool last, double dt_arg, double dt_factor,
-                             int use_ctrl, int yH, int xrows, hipStream_t st)
+                             int use_ctrl, int yH, int xrows, hipStream_t st, Stamps ts)
 {
-    return launch_sweep_t<true>(g, model, axis, last, dt_arg, dt_factor, use_ctrl, yH, xrows, st);
+    return launch_sweep_t<true>(g, model, axis, last, dt_arg, dt_factor, use_ctrl, yH, xrows, st, ts);
 }
 hipError_t launch_reduce_dt_fast(const GridView& g, int model, double* partials, hipStream_t st)
 {
@@ -314,10 +328,10 @@ hipError_t launch_reduce_dt_fast(const GridView& g, int model, double* partials,
 }
 #else
 hipError_t launch_sweep(const GridView& g, int model, int axis, bool last, double dt_arg,
-                        double dt_factor, int use_ctrl, int yH, int xrows, hipStream_t st)
+                        double dt_factor, int use_ctrl, int yH, int xrows, hipStream_t st, Stamps ts)
 {
-    if (g.opt.fast) return launch_sweep_fast(g, model, axis, last, dt_arg, dt_factor, use_ctrl, yH, xrows, st);
-    return launch_sweep_t<false>(g, model, axis, last, dt_arg, dt_factor, use_ctrl, yH, xrows, st);
+    if (g.opt.fast) return launch_sweep_fast(g, model, axis, last, dt_arg, dt_factor, use_ctrl, yH, xrows, st, ts);
+    return launch_sweep_t<false>(g, model, axis, last, dt_arg, dt_factor, use_ctrl, yH, xrows, st, ts);
 }
 hipError_t launch_reduce_dt(const GridView& g, int model, double* partials, hipStream_t st)
 {
