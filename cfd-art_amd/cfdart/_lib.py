@@ -114,6 +114,7 @@ def lib():
                                                  C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, P(C.c_int)]
         L.cfdart_batch_poll.argtypes = [C.c_void_p, P(C.c_int), P(C.c_int)]
         L.cfdart_batch_checksum.argtypes = [C.c_void_p, C.c_int, P(C.c_ulonglong)]
+        L.cfdart_batch_checksum_async.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.cfdart_batch_remove.argtypes = [C.c_void_p, C.c_int]
         L.cfdart_batch_step.argtypes = [C.c_void_p, C.c_int]
         L.cfdart_batch_download.argtypes = [C.c_void_p, C.c_int, dp]

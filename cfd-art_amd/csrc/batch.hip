@@ -52,6 +52,18 @@ __global__ void batch_finalize_kernel(BatchGrid* grids, int cap, int* status)
     status[2 * i + 1] = (c->active ? 1 : 0) | (c->error ? 2 : 0);
 }
 
+// Removal of a grid from the device table (stream-ordered: sweeps queued before it still see the
+// grid, whose finished loop makes its tiles exit at once).
+__global__ void batch_mark_dead_kernel(BatchGrid* bg)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) bg->live = 0;
+}
+
+__global__ void batch_zero_kernel(unsigned long long* p)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
+}
+
 __global__ void batch_set_lengths_kernel(BatchQueue* q, BatchQueue lengths)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) *q = lengths;
@@ -75,6 +87,14 @@ __global__ void batch_admit_kernel(BatchGrid* bg, int* status)
 using cfd::BatchGrid;
 using cfd::BatchQueue;
 using cfd::Tile;
+
+struct AdmitStage {
+    BatchGrid* rec;   // the grid record copied into the device table
+    void* draws;      // the initial condition's draws (IC_DRAW_BYTES)
+    hipEvent_t done;  // recorded after the entry's copies were queued
+    bool pending;
+};
+constexpr int ADMIT_STAGES = 32;
 
 struct FrameRegion {
     size_t off, len;  // doubles
@@ -105,6 +125,15 @@ struct cfdart_batch {
     bool tiles_dirty;        // the work-list is rebuilt before the next step
     void* d_draws;           // initial-condition draws of the admission in progress
     unsigned long long* d_sum;
+    // pinned host staging of generated admissions (grid record and draws), reused cyclically once
+    // the stream has passed an entry's copies: the admission returns without waiting
+    std::vector<AdmitStage> stages;
+    size_t stage_next;
+    // pinned host staging of the tile work-lists, and the event after their last upload
+    Tile* h_tiles[2][2];
+    size_t h_tile_cap[2][2];
+    hipEvent_t tiles_ev;
+    bool tiles_ev_pending;
     // frame staging ring in device memory: regions still being copied to the host
     double* d_ring;
     size_t ring_cap, ring_head;  // in doubles
@@ -165,34 +194,44 @@ int rebuild_tiles(cfdart_batch* b)
     }
     b->ntiles = 0;
     BatchQueue lengths{};
-    std::vector<Tile> lists[2][2];  // the host side of the copies below, kept until they finish
+    // the pinned staging of the previous upload is rewritten only once that upload has finished
+    if (b->tiles_ev_pending) {
+        BCHK(hipEventSynchronize(b->tiles_ev));
+        b->tiles_ev_pending = false;
+    }
     for (int m = 0; m < 2; ++m)
         for (int ax = 0; ax < 2; ++ax) {
             auto& l = v[m][ax];
             std::stable_sort(l.begin(), l.end(), [](const std::pair<double, Tile>& x, const std::pair<double, Tile>& y) {
                 return x.first > y.first;
             });
-            std::vector<Tile>& t = lists[m][ax];
-            t.resize(l.size());
-            for (size_t i = 0; i < l.size(); ++i) t[i] = l[i].second;
-            if (t.size() > b->tile_cap[m][ax]) {
+            if (l.size() > b->tile_cap[m][ax]) {
                 BCHK(hipStreamSynchronize(b->st));
                 BCHK(hipStreamSynchronize(b->st2));
                 if (b->d_tiles[m][ax]) BCHK(hipFree(b->d_tiles[m][ax]));
-                b->tile_cap[m][ax] = std::max(t.size(), 2 * b->tile_cap[m][ax]);
+                b->tile_cap[m][ax] = std::max(l.size(), 2 * b->tile_cap[m][ax]);
                 BCHK(hipMalloc(&b->d_tiles[m][ax], sizeof(Tile) * b->tile_cap[m][ax]));
                 b->have_graph = false;  // the graph holds the list pointers and launch sizes
             }
-            if (!t.empty())
-                BCHK(hipMemcpyAsync(b->d_tiles[m][ax], t.data(), sizeof(Tile) * t.size(), hipMemcpyHostToDevice,
-                                    b->st));
-            b->ntile[m][ax] = (int)t.size();
-            lengths.ntiles[m][ax] = (int)t.size();
-            b->ntiles += (int)t.size();
+            if (l.size() > b->h_tile_cap[m][ax]) {
+                if (b->h_tiles[m][ax]) BCHK(hipHostFree(b->h_tiles[m][ax]));
+                b->h_tiles[m][ax] = nullptr;
+                b->h_tile_cap[m][ax] = 0;
+                BCHK(hipHostMalloc((void**)&b->h_tiles[m][ax], sizeof(Tile) * b->tile_cap[m][ax], hipHostMallocDefault));
+                b->h_tile_cap[m][ax] = b->tile_cap[m][ax];
+            }
+            Tile* t = b->h_tiles[m][ax];
+            for (size_t i = 0; i < l.size(); ++i) t[i] = l[i].second;
+            if (!l.empty())
+                BCHK(hipMemcpyAsync(b->d_tiles[m][ax], t, sizeof(Tile) * l.size(), hipMemcpyHostToDevice, b->st));
+            b->ntile[m][ax] = (int)l.size();
+            lengths.ntiles[m][ax] = (int)l.size();
+            b->ntiles += (int)l.size();
         }
     hipLaunchKernelGGL(cfd::batch_set_lengths_kernel, dim3(1), dim3(64), 0, b->st, b->d_q, lengths);
     BCHK(hipGetLastError());
-    BCHK(hipStreamSynchronize(b->st));  // the host lists go out of scope
+    BCHK(hipEventRecord(b->tiles_ev, b->st));
+    b->tiles_ev_pending = true;
     return 0;
 }
 
@@ -273,13 +312,38 @@ int cfdart_batch_create(int capacity, size_t mem_budget_bytes, cfdart_batch** ou
     b->d_sum = nullptr;
     b->d_ring = nullptr;
     b->ring_cap = b->ring_head = 0;
+    b->stage_next = 0;
+    for (int m = 0; m < 2; ++m)
+        for (int ax = 0; ax < 2; ++ax) {
+            b->h_tiles[m][ax] = nullptr;
+            b->h_tile_cap[m][ax] = 0;
+        }
+    b->tiles_ev_pending = false;
+    b->tiles_ev = nullptr;
     if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&b->st2, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&b->copy_st, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&b->join_ev, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&b->join_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->tiles_ev, hipEventDisableTiming) != hipSuccess) {
         delete b;
         return bfail(CFDART_E_HIP, "stream creation failed");
+    }
+    for (int i = 0; i < ADMIT_STAGES; ++i) {
+        AdmitStage a{nullptr, nullptr, nullptr, false};
+        void* p = nullptr;
+        if (hipHostMalloc(&p, sizeof(BatchGrid) + cfd::IC_DRAW_BYTES, hipHostMallocDefault) != hipSuccess) break;
+        if (hipEventCreateWithFlags(&a.done, hipEventDisableTiming) != hipSuccess) {
+            (void)hipHostFree(p);
+            break;
+        }
+        a.rec = (BatchGrid*)p;
+        a.draws = (char*)p + sizeof(BatchGrid);
+        b->stages.push_back(a);
+    }
+    if (b->stages.size() != (size_t)ADMIT_STAGES) {
+        cfdart_batch_destroy(b);
+        return bfail(CFDART_E_NOMEM, "pinned admission staging");
     }
     if (hipMalloc(&b->d_grids, sizeof(BatchGrid) * capacity) != hipSuccess ||
         hipMalloc(&b->d_q, sizeof(BatchQueue)) != hipSuccess ||
@@ -325,6 +389,14 @@ void cfdart_batch_destroy(cfdart_batch* b)
     if (b->d_sum) (void)hipFree(b->d_sum);
     if (b->d_ring) (void)hipFree(b->d_ring);
     for (FrameRegion& r : b->inflight) (void)hipEventDestroy(r.done);
+    for (AdmitStage& a : b->stages) {
+        (void)hipEventDestroy(a.done);
+        (void)hipHostFree(a.rec);
+    }
+    for (int m = 0; m < 2; ++m)
+        for (int ax = 0; ax < 2; ++ax)
+            if (b->h_tiles[m][ax]) (void)hipHostFree(b->h_tiles[m][ax]);
+    if (b->tiles_ev) (void)hipEventDestroy(b->tiles_ev);
     (void)hipEventDestroy(b->fork_ev);
     (void)hipEventDestroy(b->join_ev);
     (void)hipStreamDestroy(b->st2);
@@ -406,6 +478,21 @@ int admit(cfdart_batch* b, const cfdart_desc* d, const double* host, const cfdar
         b->pool.insert({one, g.buf[0]});
         b->pool.insert({one, g.buf[1]});
     };
+    // a generated state is admitted without waiting: its draws and grid record go through a
+    // pinned staging entry, reused once the stream has passed that entry's copies
+    AdmitStage* stage = nullptr;
+    if (!host) {
+        stage = &b->stages[b->stage_next];
+        b->stage_next = (b->stage_next + 1) % b->stages.size();
+        if (stage->pending) {
+            hipError_t e = hipEventSynchronize(stage->done);
+            if (e != hipSuccess) {
+                release();
+                return bfail(CFDART_E_HIP, std::string("admission staging: ") + hipGetErrorString(e));
+            }
+            stage->pending = false;
+        }
+    }
     std::vector<double> soa;
     if (host) {
         // interior of the host grid -> SoA buf[0]
@@ -422,13 +509,18 @@ int admit(cfdart_batch* b, const cfdart_desc* d, const double* host, const cfdar
         }
     } else {
         std::string err;
-        int rc = cfd::generate_state(*d, dp, bp, g.buf[0], pitch, plane, b->d_draws, b->st, err);
+        int rc = cfd::generate_state(*d, dp, bp, g.buf[0], pitch, plane, b->d_draws, b->st, err, stage->draws);
         if (rc) {
             release();
             return bfail(rc, err);
         }
     }
-    hipError_t e = hipMemcpyAsync(b->d_grids + slot, &g, sizeof(BatchGrid), hipMemcpyHostToDevice, b->st);
+    const BatchGrid* rec = &g;
+    if (stage) {
+        *stage->rec = g;
+        rec = stage->rec;
+    }
+    hipError_t e = hipMemcpyAsync(b->d_grids + slot, rec, sizeof(BatchGrid), hipMemcpyHostToDevice, b->st);
     if (e != hipSuccess) {
         release();
         return bfail(CFDART_E_HIP, std::string("grid table upload: ") + hipGetErrorString(e));
@@ -455,7 +547,14 @@ int admit(cfdart_batch* b, const cfdart_desc* d, const double* host, const cfdar
                            b->d_status + 2 * slot);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(b->st);  // the staging vector and g go out of scope
+    if (e == hipSuccess) {
+        if (stage) {
+            e = hipEventRecord(stage->done, b->st);
+            stage->pending = e == hipSuccess;
+        } else {
+            e = hipStreamSynchronize(b->st);  // the staging vector and g go out of scope
+        }
+    }
     if (e != hipSuccess) {
         release();
         return bfail(CFDART_E_HIP, std::string("admission kernels: ") + hipGetErrorString(e));
@@ -493,11 +592,12 @@ int cfdart_batch_remove(cfdart_batch* b, int slot)
     if (rc) return rc;
     BatchGrid& g = b->h_grids[slot];
     g.live = 0;
-    // mark dead on the device before the buffers can be reused
-    BCHK(hipMemcpyAsync(&(b->d_grids + slot)->live, &g.live, sizeof(int), hipMemcpyHostToDevice, b->st));
+    // marked dead on the device in stream order: work queued later (an admission reusing the
+    // buffers, the next steps) sees it dead, work queued earlier still sees its state
+    hipLaunchKernelGGL(cfd::batch_mark_dead_kernel, dim3(1), dim3(64), 0, b->st, b->d_grids + slot);
+    BCHK(hipGetLastError());
     b->live[slot] = 0;
     b->tiles_dirty = true;
-    BCHK(hipStreamSynchronize(b->st));
     size_t one = b->bytes[slot] / 2;
     b->pool.insert({one, g.buf[0]});
     b->pool.insert({one, g.buf[1]});
@@ -723,6 +823,19 @@ int cfdart_batch_checksum(cfdart_batch* b, int slot, unsigned long long* out)
     BCHK(cfd::launch_checksum(g.buf[0], g.pitch, g.plane, g.nx, g.ny, g.model == cfd::ALLAIRE ? 6 : 4, b->d_sum, b->st));
     BCHK(hipMemcpyAsync(out, b->d_sum, sizeof(unsigned long long), hipMemcpyDeviceToHost, b->st));
     BCHK(hipStreamSynchronize(b->st));
+    return 0;
+}
+
+int cfdart_batch_checksum_async(cfdart_batch* b, int slot, unsigned long long* host_out)
+{
+    int rc = check_slot(b, slot);
+    if (rc) return rc;
+    if (!host_out) return bfail(CFDART_E_ARG, "null output");
+    const BatchGrid& g = b->h_grids[slot];
+    hipLaunchKernelGGL(cfd::batch_zero_kernel, dim3(1), dim3(64), 0, b->st, b->d_sum);
+    BCHK(hipGetLastError());
+    BCHK(cfd::launch_checksum(g.buf[0], g.pitch, g.plane, g.nx, g.ny, g.model == cfd::ALLAIRE ? 6 : 4, b->d_sum, b->st));
+    BCHK(hipMemcpyAsync(host_out, b->d_sum, sizeof(unsigned long long), hipMemcpyDeviceToHost, b->st));
     return 0;
 }
 

@@ -33,8 +33,12 @@ constexpr size_t IC_DRAW_BYTES = sizeof(Disc) * IC_MAX_DISCS;
 
 // Fills the interior planes of buf from exactly one of dp (Euler random discs) or bp (Allaire
 // shocked bubbles): host draws, copied into d_draws (IC_DRAW_BYTES of device memory), then the
-// device generator on stream st, which is synchronised before returning. 0 or CFDART_E_*.
+// device generator on stream st. Without h_stage the draws are copied from the stack and the
+// stream is synchronised before returning; with h_stage (IC_DRAW_BYTES of pinned host memory)
+// the draws are copied from there and nothing waits: the caller keeps h_stage unchanged until the
+// stream has passed the copy. 0 or CFDART_E_*.
 int generate_state(const cfdart_desc& d, const cfdart_disc_params* dp, const cfdart_bubble_params* bp, double* buf,
-                   int pitch, long long plane, void* d_draws, hipStream_t st, std::string& err);
+                   int pitch, long long plane, void* d_draws, hipStream_t st, std::string& err,
+                   void* h_stage = nullptr);
 
 }  // namespace cfd

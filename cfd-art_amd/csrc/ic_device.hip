@@ -11,6 +11,7 @@
 
 #include "ic_common.h"
 
+#include <cstring>
 #include <string>
 
 namespace cfd {
@@ -148,7 +149,7 @@ hipError_t launch_checksum(const double* buf, int pitch, long long plane, int nx
 }
 
 int generate_state(const cfdart_desc& d, const cfdart_disc_params* dp, const cfdart_bubble_params* bp, double* buf,
-                   int pitch, long long plane, void* d_draws, hipStream_t st, std::string& err)
+                   int pitch, long long plane, void* d_draws, hipStream_t st, std::string& err, void* h_stage)
 {
     if ((dp != nullptr) == (bp != nullptr)) {
         err = "give exactly one of the disc and bubble parameters";
@@ -171,18 +172,22 @@ int generate_state(const cfdart_desc& d, const cfdart_disc_params* dp, const cfd
         }
         double Wa[4] = {dp->ambient[0], dp->ambient[1], dp->ambient[2], dp->ambient[3]}, Ua[4];
         p2c<EULER>(make_material(d.gamma1, d.pinf1, d.gamma2, d.pinf2), Wa, Ua);
-        e = hipMemcpyAsync(d_draws, h, sizeof(Disc) * (size_t)dp->n, hipMemcpyHostToDevice, st);
+        const void* src = h;
+        if (h_stage) src = std::memcpy(h_stage, h, sizeof(Disc) * (size_t)dp->n);
+        e = hipMemcpyAsync(d_draws, src, sizeof(Disc) * (size_t)dp->n, hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = launch_ic_discs(buf, pitch, plane, d, Ua, (const Disc*)d_draws, dp->n, st);
-        if (e == hipSuccess) e = hipStreamSynchronize(st);  // h goes out of scope
+        if (e == hipSuccess && !h_stage) e = hipStreamSynchronize(st);  // h goes out of scope
     } else {
         Bubble h[IC_MAX_DISCS];
         if (bp->n < 0 || bp->n > IC_MAX_DISCS || draw_bubbles(bp, h, IC_MAX_DISCS) != 0) {
             err = "bad bubble parameters";
             return CFDART_E_ARG;
         }
-        e = hipMemcpyAsync(d_draws, h, sizeof(Bubble) * (size_t)bp->n, hipMemcpyHostToDevice, st);
+        const void* src = h;
+        if (h_stage) src = std::memcpy(h_stage, h, sizeof(Bubble) * (size_t)bp->n);
+        e = hipMemcpyAsync(d_draws, src, sizeof(Bubble) * (size_t)bp->n, hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = launch_ic_bubbles(buf, pitch, plane, d, *bp, (const Bubble*)d_draws, bp->n, st);
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e == hipSuccess && !h_stage) e = hipStreamSynchronize(st);
     }
     if (e != hipSuccess) {
         err = std::string("initial-condition kernel: ") + hipGetErrorString(e);

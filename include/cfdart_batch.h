@@ -47,11 +47,14 @@ int cfdart_batch_add(cfdart_batch* b, const cfdart_desc* desc, const double* hos
                      int* slot_out);
 /* As cfdart_batch_add with the initial state generated on the device from exactly one of
  * discs (Euler, cfdart_ic_random_discs) or bubbles (Allaire, cfdart_ic_shocked_bubbles);
- * the other is NULL. Bitwise the state the host generator would upload. */
+ * the other is NULL. Bitwise the state the host generator would upload. Returns once the
+ * generation and the first dt are queued on the batch stream, without waiting for the device. */
 int cfdart_batch_add_generated(cfdart_batch* b, const cfdart_desc* desc, const cfdart_disc_params* discs,
                                const cfdart_bubble_params* bubbles, int splitting, int max_steps, double T,
                                double cfl, int y_strip_rows, int x_rows_per_wave, int* slot_out);
-/* Removes a grid (finished or not) and frees its device memory for reuse. */
+/* Removes a grid (finished or not) and frees its device memory for reuse, in stream order and
+ * without waiting: work queued earlier still sees the grid, work queued later (steps, an
+ * admission that reuses its memory) does not. */
 int cfdart_batch_remove(cfdart_batch* b, int slot);
 /* Advances every live grid by nsteps steps (grids that have finished stay unchanged). */
 int cfdart_batch_step(cfdart_batch* b, int nsteps);
@@ -66,6 +69,10 @@ int cfdart_batch_status(cfdart_batch* b, int slot, double* t, int* n, int* activ
 int cfdart_batch_poll(cfdart_batch* b, int* steps, int* flags);
 /* Checksum of a grid's current state, as cfdart_checksum; synchronous. */
 int cfdart_batch_checksum(cfdart_batch* b, int slot, unsigned long long* out);
+/* The same checksum queued on the batch stream without waiting: *host_out (pinned memory,
+ * cfdart_host_alloc) holds it after the next cfdart_batch_poll or cfdart_batch_sync. The state
+ * is the grid's state at that point of the stream (the last step queued before the call). */
+int cfdart_batch_checksum_async(cfdart_batch* b, int slot, unsigned long long* host_out);
 /* Fallback counters of a grid (see CFDART_CNT_*), 4 values; synchronous. */
 int cfdart_batch_stats(cfdart_batch* b, int slot, long long* counters);
 int cfdart_batch_sync(cfdart_batch* b);

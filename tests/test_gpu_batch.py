@@ -167,3 +167,41 @@ def test_batch_refuses_what_create_refuses(gpu):
     assert e.value.code == -1
     assert b.info()[0] == 0
     b.close()
+
+
+def test_async_admission_removal_and_checksum(gpu):
+    """Generated admissions, removals and checksums queued without waiting: a grid removed while
+    steps are still queued keeps its state for them, a generated grid admitted right after into its
+    memory starts from its own initial state, and a checksum queued between steps reports the
+    state at that point of the stream (read after the next poll), whatever is queued after it."""
+    import ctypes as C
+    L = cfdart.lib()
+    d, p = cfdart.configs.euler_discs_params(96, 4, (0.05, 0.15), seed=11)
+    d2, p2 = cfdart.configs.euler_discs_params(96, 4, (0.05, 0.15), seed=12)
+    b = cfdart.Batch(4)
+    s0 = b.add_generated(d, p, max_steps=30, cfl=0.8)
+    b.step(6)
+    buf = L.cfdart_host_alloc(8)
+    assert buf
+    try:
+        cfdart._lib._check(L.cfdart_batch_checksum_async(b.h, s0, C.c_void_p(buf)))
+        b.step(4)                       # queued after the checksum
+        b.remove(s0)                    # queued after those steps
+        s1 = b.add_generated(d2, p2, max_steps=5, cfl=0.8)  # may reuse s0's slot and memory
+        b.step(5)
+        steps, flags = b.poll(4)
+        print(f"case async batch ops: slots {s0} then {s1}", flush=True)
+        got = C.c_ulonglong.from_address(buf).value
+    finally:
+        L.cfdart_host_free(C.c_void_p(buf))
+    h = cfdart.random_discs(d, p)
+    g = cfdart.Grid(d)
+    g.upload(h)
+    g.run(cfdart.ALTERNATING, 6, cfl=0.8)
+    assert got == g.checksum()
+    g.close()
+    h2 = cfdart.random_discs(d2, p2)
+    ref2, _, n2 = alone(d2, h2, 0.8, 5)
+    assert steps[s1] == n2 == 5 and flags[s1] == 0
+    assert bitwise_equal(interior(b.download(s1)), interior(ref2))
+    b.close()
