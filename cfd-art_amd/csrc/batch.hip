@@ -174,23 +174,36 @@ double* pool_get(cfdart_batch* b, size_t bytes)
     return p;
 }
 
-// Estimated relative cost of an x tile (LPT ordering: heavy tiles first; y tiles are all alike).
-double tile_cost(const BatchGrid& g) { return 60.0 * 4 * g.xrows; }
-
+// The work-lists of the live grids, per model and axis. x tiles go largest first (longest
+// processing time first: decreasing rows per wavefront, grids in slot order within a shape), which
+// shortens the tail of the launch; y tiles are all alike and go in slot order, each grid's column
+// blocks in the outer loop so that tiles sharing halo rows are consecutive. The order is built in
+// one pass per distinct tile shape (no comparison sort: a batch rebuilds its lists at every
+// admission and removal).
 int rebuild_tiles(cfdart_batch* b)
 {
-    std::vector<std::pair<double, Tile>> v[2][2];
+    std::vector<Tile> v[2][2];
+    std::vector<int> shapes[2];  // distinct x rows per wavefront, per model
     for (int s = 0; s < b->cap; ++s) {
         if (!b->live[s]) continue;
         const BatchGrid& g = b->h_grids[s];
         const int m = g.model == cfd::ALLAIRE ? 1 : 0;
-        const int ntx = (g.nx + cfd::XTILE - 1) / cfd::XTILE, nry = (g.ny + 4 * g.xrows - 1) / (4 * g.xrows);
-        for (int a = 0; a < ntx; ++a)
-            for (int r = 0; r < nry; ++r) v[m][0].push_back({tile_cost(g), Tile{s, 0, a, r}});
-        // column blocks in the outer loop: tiles sharing halo rows are consecutive
+        if (std::find(shapes[m].begin(), shapes[m].end(), g.xrows) == shapes[m].end()) shapes[m].push_back(g.xrows);
         const int nbx = (g.nx + cfd::BATCH_YT_COLS - 1) / cfd::BATCH_YT_COLS, nby = (g.ny + cfd::YT_ROWS - 1) / cfd::YT_ROWS;
         for (int a = 0; a < nbx; ++a)
-            for (int r = 0; r < nby; ++r) v[m][1].push_back({0.0, Tile{s, 1, a, r}});
+            for (int r = 0; r < nby; ++r) v[m][1].push_back(Tile{s, 1, a, r});
+    }
+    for (int m = 0; m < 2; ++m) {
+        std::sort(shapes[m].begin(), shapes[m].end(), [](int x, int y) { return x > y; });
+        for (int xr : shapes[m])
+            for (int s = 0; s < b->cap; ++s) {
+                if (!b->live[s]) continue;
+                const BatchGrid& g = b->h_grids[s];
+                if ((g.model == cfd::ALLAIRE ? 1 : 0) != m || g.xrows != xr) continue;
+                const int ntx = (g.nx + cfd::XTILE - 1) / cfd::XTILE, nry = (g.ny + 4 * g.xrows - 1) / (4 * g.xrows);
+                for (int a = 0; a < ntx; ++a)
+                    for (int r = 0; r < nry; ++r) v[m][0].push_back(Tile{s, 0, a, r});
+            }
     }
     b->ntiles = 0;
     BatchQueue lengths{};
@@ -201,10 +214,7 @@ int rebuild_tiles(cfdart_batch* b)
     }
     for (int m = 0; m < 2; ++m)
         for (int ax = 0; ax < 2; ++ax) {
-            auto& l = v[m][ax];
-            std::stable_sort(l.begin(), l.end(), [](const std::pair<double, Tile>& x, const std::pair<double, Tile>& y) {
-                return x.first > y.first;
-            });
+            const std::vector<Tile>& l = v[m][ax];
             if (l.size() > b->tile_cap[m][ax]) {
                 BCHK(hipStreamSynchronize(b->st));
                 BCHK(hipStreamSynchronize(b->st2));
@@ -220,10 +230,11 @@ int rebuild_tiles(cfdart_batch* b)
                 BCHK(hipHostMalloc((void**)&b->h_tiles[m][ax], sizeof(Tile) * b->tile_cap[m][ax], hipHostMallocDefault));
                 b->h_tile_cap[m][ax] = b->tile_cap[m][ax];
             }
-            Tile* t = b->h_tiles[m][ax];
-            for (size_t i = 0; i < l.size(); ++i) t[i] = l[i].second;
-            if (!l.empty())
-                BCHK(hipMemcpyAsync(b->d_tiles[m][ax], t, sizeof(Tile) * l.size(), hipMemcpyHostToDevice, b->st));
+            if (!l.empty()) {
+                std::memcpy(b->h_tiles[m][ax], l.data(), sizeof(Tile) * l.size());
+                BCHK(hipMemcpyAsync(b->d_tiles[m][ax], b->h_tiles[m][ax], sizeof(Tile) * l.size(),
+                                    hipMemcpyHostToDevice, b->st));
+            }
             b->ntile[m][ax] = (int)l.size();
             lengths.ntiles[m][ax] = (int)l.size();
             b->ntiles += (int)l.size();
