@@ -30,14 +30,17 @@ int check_desc(const cfdart_desc* d);         // cfdart_api.hip: cfdart_create's
 namespace cfd {
 
 // End of a step for every live grid (E/headers/simulation.hpp:56-58 per grid, plus the
-// step budget).
-__global__ void batch_finalize_kernel(BatchGrid* grids, int cap, int* status)
+// step budget): one wavefront per grid slot, lane i folding wave-speed slot i (one memory round
+// trip for the 64 slots instead of a serial fold in one thread).
+__global__ __launch_bounds__(64) void batch_finalize_kernel(BatchGrid* grids, int* status)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= cap) return;
+    const int i = blockIdx.x;
     BatchGrid* bg = grids + i;
     Ctrl* c = &bg->ctrl;
-    if (!bg->live || !c->active) return;
+    if (!bg->live || !c->active) return;  // the same in every lane
+    double mu, mv;
+    fold_maxima_wave(c, mu, mv);  // the whole wavefront
+    if (threadIdx.x != 0) return;
     double t = c->t + c->dt;
     int n = c->n + 1;
     if (fabs(t - c->T) < 1e-12) t = c->T;
@@ -45,8 +48,6 @@ __global__ void batch_finalize_kernel(BatchGrid* grids, int cap, int* status)
     c->n = n;
     bg->steps_left -= 1;
     c->active = (t < c->T) && bg->steps_left > 0 && !c->error;
-    double mu, mv;
-    fold_maxima(c, mu, mv);
     if (c->active) c->dt = dt_from_maxima(c, mu, mv, n, t, bg->dx, bg->dy);
     status[2 * i] = n;
     status[2 * i + 1] = (c->active ? 1 : 0) | (c->error ? 2 : 0);
@@ -275,8 +276,7 @@ int capture_graph(cfdart_batch* b)
     }
     BCHK(hipEventRecord(b->join_ev, b->st2));
     BCHK(hipStreamWaitEvent(b->st, b->join_ev, 0));
-    hipLaunchKernelGGL(cfd::batch_finalize_kernel, dim3((b->cap + 255) / 256), dim3(256), 0, b->st, b->d_grids,
-                       b->cap, b->d_status);
+    hipLaunchKernelGGL(cfd::batch_finalize_kernel, dim3(b->cap), dim3(64), 0, b->st, b->d_grids, b->d_status);
     BCHK(hipStreamEndCapture(b->st, &graph));
     hipError_t e = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
