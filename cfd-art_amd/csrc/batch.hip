@@ -117,8 +117,11 @@ struct cfdart_batch {
     int nred;
     hipStream_t st, st2, copy_st;
     hipEvent_t fork_ev, join_ev;
-    hipGraphExec_t exec;
+    hipGraphExec_t exec[cfd::STEP_SHAPES];  // the captured step, per shape
     bool have_graph;
+    long long gstep;            // batch steps launched since creation
+    std::vector<int> cls;       // per slot: 0 / 1 alternating grid admitted at an even / odd step, 2 naive
+    int nclass[3];              // live grids per class
     int arith;  // CFDART_ARITH_EXACT or CFDART_ARITH_FAST, for every grid of the batch
     int ntiles;
     std::multimap<size_t, double*> pool;  // freed state buffers for reuse, by size
@@ -175,6 +178,14 @@ double* pool_get(cfdart_batch* b, size_t bytes)
     return p;
 }
 
+// The captured steps (they hold the work-lists' addresses and launch sizes).
+void destroy_graphs(cfdart_batch* b)
+{
+    if (!b->have_graph) return;
+    for (int k = 0; k < cfd::STEP_SHAPES; ++k) (void)hipGraphExecDestroy(b->exec[k]);
+    b->have_graph = false;
+}
+
 // The work-lists of the live grids, per model and axis. x tiles go largest first (longest
 // processing time first: decreasing rows per wavefront, grids in slot order within a shape), which
 // shortens the tail of the launch; y tiles are all alike and go in slot order, each grid's column
@@ -222,7 +233,7 @@ int rebuild_tiles(cfdart_batch* b)
                 if (b->d_tiles[m][ax]) BCHK(hipFree(b->d_tiles[m][ax]));
                 b->tile_cap[m][ax] = std::max(l.size(), 2 * b->tile_cap[m][ax]);
                 BCHK(hipMalloc(&b->d_tiles[m][ax], sizeof(Tile) * b->tile_cap[m][ax]));
-                b->have_graph = false;  // the graph holds the list pointers and launch sizes
+                destroy_graphs(b);  // re-captured with the new list and launch size
             }
             if (l.size() > b->h_tile_cap[m][ax]) {
                 if (b->h_tiles[m][ax]) BCHK(hipHostFree(b->h_tiles[m][ax]));
@@ -248,41 +259,56 @@ int rebuild_tiles(cfdart_batch* b)
 }
 
 // One step = both sweep phases of the Euler and the Allaire kernels (two concurrent branches),
-// joined before the per-grid loop bookkeeping.
+// joined before the per-grid loop bookkeeping; captured once per step shape (cfd::StepShape).
 int capture_graph(cfdart_batch* b)
 {
-    if (b->have_graph) {
-        (void)hipGraphExecDestroy(b->exec);
-        b->have_graph = false;
-    }
+    destroy_graphs(b);
     for (int m = 0; m < 2; ++m)
         for (int ax = 0; ax < 2; ++ax)
             if (!b->d_tiles[m][ax]) {
                 b->tile_cap[m][ax] = 1;
                 BCHK(hipMalloc(&b->d_tiles[m][ax], sizeof(Tile)));
             }
-    hipGraph_t graph;
-    BCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-    BCHK(hipEventRecord(b->fork_ev, b->st));
-    BCHK(hipStreamWaitEvent(b->st2, b->fork_ev, 0));
     using cfd::ALLAIRE;
     using cfd::EULER;
-    if (b->arith == CFDART_ARITH_FAST) {
-        cfd::launch_batch_phases_fast(EULER, b->d_grids, b->d_tiles[EULER], b->tile_cap[EULER], b->d_q, b->st);
-        cfd::launch_batch_phases_fast(ALLAIRE, b->d_grids, b->d_tiles[ALLAIRE], b->tile_cap[ALLAIRE], b->d_q, b->st2);
-    } else {
-        cfd::launch_batch_phases<EULER, false>(b->d_grids, b->d_tiles[EULER], b->tile_cap[EULER], b->d_q, b->st);
-        cfd::launch_batch_phases<ALLAIRE, false>(b->d_grids, b->d_tiles[ALLAIRE], b->tile_cap[ALLAIRE], b->d_q, b->st2);
+    for (int k = 0; k < cfd::STEP_SHAPES; ++k) {
+        hipGraph_t graph;
+        BCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+        BCHK(hipEventRecord(b->fork_ev, b->st));
+        BCHK(hipStreamWaitEvent(b->st2, b->fork_ev, 0));
+        if (b->arith == CFDART_ARITH_FAST) {
+            cfd::launch_batch_phases_fast(EULER, b->d_grids, b->d_tiles[EULER], b->tile_cap[EULER], b->d_q, b->st, k);
+            cfd::launch_batch_phases_fast(ALLAIRE, b->d_grids, b->d_tiles[ALLAIRE], b->tile_cap[ALLAIRE], b->d_q, b->st2, k);
+        } else {
+            cfd::launch_batch_phases<EULER, false>(b->d_grids, b->d_tiles[EULER], b->tile_cap[EULER], b->d_q, b->st, k);
+            cfd::launch_batch_phases<ALLAIRE, false>(b->d_grids, b->d_tiles[ALLAIRE], b->tile_cap[ALLAIRE], b->d_q, b->st2,
+                                                     k);
+        }
+        BCHK(hipEventRecord(b->join_ev, b->st2));
+        BCHK(hipStreamWaitEvent(b->st, b->join_ev, 0));
+        hipLaunchKernelGGL(cfd::batch_finalize_kernel, dim3(b->cap), dim3(64), 0, b->st, b->d_grids, b->d_status);
+        BCHK(hipStreamEndCapture(b->st, &graph));
+        hipError_t e = hipGraphInstantiate(&b->exec[k], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            for (int j = 0; j < k; ++j) (void)hipGraphExecDestroy(b->exec[j]);
+            return bfail(CFDART_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
     }
-    BCHK(hipEventRecord(b->join_ev, b->st2));
-    BCHK(hipStreamWaitEvent(b->st, b->join_ev, 0));
-    hipLaunchKernelGGL(cfd::batch_finalize_kernel, dim3(b->cap), dim3(64), 0, b->st, b->d_grids, b->d_status);
-    BCHK(hipStreamEndCapture(b->st, &graph));
-    hipError_t e = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (e != hipSuccess) return bfail(CFDART_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     b->have_graph = true;
     return 0;
+}
+
+// The shape of batch step number s (counted from the batch's creation): an alternating grid
+// admitted at step g has taken s - g steps, so it sweeps x first when s + g is even; a naive grid
+// always sweeps x first. When every admitted grid sweeps the same axis first, the step launches
+// only the sweeps that have work (a grid that has finished exits at once whatever the shape).
+int step_shape(const cfdart_batch* b, long long s)
+{
+    const bool x_first = b->nclass[2] > 0 || (b->nclass[0] > 0 && (s & 1) == 0) || (b->nclass[1] > 0 && (s & 1) == 1);
+    const bool y_first = (b->nclass[0] > 0 && (s & 1) == 1) || (b->nclass[1] > 0 && (s & 1) == 0);
+    if (x_first && y_first) return cfd::STEP_BOTH;
+    return y_first ? cfd::STEP_YX : cfd::STEP_XY;
 }
 
 int check_slot(cfdart_batch* b, int slot)
@@ -315,6 +341,9 @@ int cfdart_batch_create(int capacity, size_t mem_budget_bytes, cfdart_batch** ou
             b->ntile[m][ax] = 0;
         }
     b->have_graph = false;
+    b->gstep = 0;
+    b->cls.assign(capacity, 0);
+    for (int k = 0; k < 3; ++k) b->nclass[k] = 0;
     b->arith = CFDART_ARITH_EXACT;
     b->ntiles = 0;
     b->d_status = nullptr;
@@ -388,7 +417,7 @@ void cfdart_batch_destroy(cfdart_batch* b)
             (void)hipFree(b->h_grids[s].buf[1]);
         }
     for (auto& kv : b->pool) (void)hipFree(kv.second);
-    if (b->have_graph) (void)hipGraphExecDestroy(b->exec);
+    destroy_graphs(b);
     for (int m = 0; m < 2; ++m)
         for (int ax = 0; ax < 2; ++ax)
             if (b->d_tiles[m][ax]) (void)hipFree(b->d_tiles[m][ax]);
@@ -572,6 +601,8 @@ int admit(cfdart_batch* b, const cfdart_desc* d, const double* host, const cfdar
     }
     b->h_grids[slot] = g;
     b->live[slot] = 1;
+    b->cls[slot] = splitting == CFDART_ALTERNATING ? (int)(b->gstep & 1) : 2;
+    b->nclass[b->cls[slot]] += 1;
     b->bytes[slot] = need;
     b->used += need;
     b->tiles_dirty = true;
@@ -608,6 +639,7 @@ int cfdart_batch_remove(cfdart_batch* b, int slot)
     hipLaunchKernelGGL(cfd::batch_mark_dead_kernel, dim3(1), dim3(64), 0, b->st, b->d_grids + slot);
     BCHK(hipGetLastError());
     b->live[slot] = 0;
+    b->nclass[b->cls[slot]] -= 1;
     b->tiles_dirty = true;
     size_t one = b->bytes[slot] / 2;
     b->pool.insert({one, g.buf[0]});
@@ -630,7 +662,10 @@ int cfdart_batch_step(cfdart_batch* b, int nsteps)
         int rc = capture_graph(b);
         if (rc) return rc;
     }
-    for (int i = 0; i < nsteps; ++i) BCHK(hipGraphLaunch(b->exec, b->st));
+    for (int i = 0; i < nsteps; ++i) {
+        BCHK(hipGraphLaunch(b->exec[step_shape(b, b->gstep)], b->st));
+        ++b->gstep;
+    }
     return 0;
 }
 
@@ -721,10 +756,9 @@ int cfdart_batch_set_arith(cfdart_batch* b, int mode)
     if (!b || (mode != CFDART_ARITH_EXACT && mode != CFDART_ARITH_FAST)) return bfail(CFDART_E_ARG, "bad batch or arithmetic mode");
     if (mode == b->arith) return 0;
     b->arith = mode;
-    if (b->have_graph) {  // the captured step holds the other mode's kernels
+    if (b->have_graph) {  // the captured steps hold the other mode's kernels
         BCHK(hipStreamSynchronize(b->st));
-        (void)hipGraphExecDestroy(b->exec);
-        b->have_graph = false;
+        destroy_graphs(b);
     }
     return 0;
 }

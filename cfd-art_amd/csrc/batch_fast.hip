@@ -9,10 +9,10 @@
 namespace cfd {
 
 void launch_batch_phases_fast(int model, BatchGrid* grids, Tile* const* tiles, const size_t* cap, const BatchQueue* q,
-                              hipStream_t st)
+                              hipStream_t st, int shape)
 {
-    if (model == EULER) launch_batch_phases<EULER, true>(grids, tiles, cap, q, st);
-    else launch_batch_phases<ALLAIRE, true>(grids, tiles, cap, q, st);
+    if (model == EULER) launch_batch_phases<EULER, true>(grids, tiles, cap, q, st, shape);
+    else launch_batch_phases<ALLAIRE, true>(grids, tiles, cap, q, st, shape);
 }
 
 }  // namespace cfd

@@ -97,20 +97,26 @@ __global__ __launch_bounds__(256) void batch_y_kernel(BatchGrid* grids, const Ti
     ta.flush(&bg->ctrl, LAST, t);
 }
 
-// The x and y kernels of model M for both sweep phases of a step, on one stream (each kernel's
-// grid is its work-list's capacity; items past the live count exit at once).
+// Which sweeps a captured step launches: every axis in both phases (grids of the batch sweep
+// different axes in a phase), or only x then y, or only y then x (every running grid of the batch
+// sweeps that axis in that phase: the other launches would be all exits).
+enum StepShape { STEP_BOTH = 0, STEP_XY = 1, STEP_YX = 2, STEP_SHAPES = 3 };
+
+// The x and y kernels of model M for the sweep phases of a step of the given shape, on one stream
+// (each kernel's grid is its work-list's capacity; items past the live count exit at once).
 template <int M, bool F>
-void launch_batch_phases(BatchGrid* grids, Tile* const* tiles, const size_t* cap, const BatchQueue* q, hipStream_t st)
+void launch_batch_phases(BatchGrid* grids, Tile* const* tiles, const size_t* cap, const BatchQueue* q, hipStream_t st,
+                         int shape)
 {
     const dim3 bx((unsigned)cap[0]), by((unsigned)cap[1]);
-    hipLaunchKernelGGL((batch_x_kernel<M, false, F>), bx, dim3(256), 0, st, grids, tiles[0], q, 0);
-    hipLaunchKernelGGL((batch_y_kernel<M, false, F>), by, dim3(256), 0, st, grids, tiles[1], q, 0);
-    hipLaunchKernelGGL((batch_x_kernel<M, true, F>), bx, dim3(256), 0, st, grids, tiles[0], q, 1);
-    hipLaunchKernelGGL((batch_y_kernel<M, true, F>), by, dim3(256), 0, st, grids, tiles[1], q, 1);
+    if (shape != STEP_YX) hipLaunchKernelGGL((batch_x_kernel<M, false, F>), bx, dim3(256), 0, st, grids, tiles[0], q, 0);
+    if (shape != STEP_XY) hipLaunchKernelGGL((batch_y_kernel<M, false, F>), by, dim3(256), 0, st, grids, tiles[1], q, 0);
+    if (shape != STEP_XY) hipLaunchKernelGGL((batch_x_kernel<M, true, F>), bx, dim3(256), 0, st, grids, tiles[0], q, 1);
+    if (shape != STEP_YX) hipLaunchKernelGGL((batch_y_kernel<M, true, F>), by, dim3(256), 0, st, grids, tiles[1], q, 1);
 }
 
 // The fast arithmetic's phases (batch_fast.hip).
 void launch_batch_phases_fast(int model, BatchGrid* grids, Tile* const* tiles, const size_t* cap, const BatchQueue* q,
-                              hipStream_t st);
+                              hipStream_t st, int shape);
 
 }  // namespace cfd
