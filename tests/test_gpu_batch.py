@@ -205,3 +205,31 @@ def test_async_admission_removal_and_checksum(gpu):
     assert steps[s1] == n2 == 5 and flags[s1] == 0
     assert bitwise_equal(interior(b.download(s1)), interior(ref2))
     b.close()
+
+
+def test_step_shapes_follow_each_grid_parity(gpu):
+    """A batch launches only the sweeps with work when every grid sweeps the same axis first (its
+    step shape follows the grids' admission steps): a grid admitted at an odd batch step, left
+    alone once the grids admitted at even steps are removed, still ends bitwise where it ends alone;
+    so does a grid admitted later at an even step beside it (both shapes in one batch)."""
+    cases = mixed_cases(4, 4)
+    b = cfdart.Batch(4)
+    (d0, h0, c0, _), (d1, h1, c1, _), (d2, h2, c2, _) = cases[:3]
+    s0 = b.add(d0, h0, max_steps=3, cfl=c0)       # admitted at step 0
+    b.step(3)
+    s1 = b.add(d1, h1, max_steps=17, cfl=c1)      # admitted at step 3: sweeps y first on even steps
+    b.step(2)
+    b.remove(s0)                                  # only the odd-admitted grid is left
+    b.step(6)
+    s2 = b.add(d2, h2, max_steps=9, cfl=c2)       # admitted at step 11: x first on odd steps, like s1
+    b.step(5)
+    s3 = b.add(*cases[3][:2], max_steps=5, cfl=cases[3][2])  # admitted at step 16: the other parity
+    b.step(20)
+    b.sync()
+    for (d, h, cfl, steps), s in (((d1, h1, c1, 17), s1), ((d2, h2, c2, 9), s2), (tuple(cases[3][:3]) + (5,), s3)):
+        ref, t, n = alone(d, h, cfl, steps)
+        tb, nb, act = b.status(s)
+        print(f"case shapes slot={s} model={d.model} {d.nx}x{d.ny} steps={steps}", flush=True)
+        assert nb == n == steps and tb == t and not act
+        assert bitwise_equal(interior(b.download(s)), interior(ref))
+    b.close()
