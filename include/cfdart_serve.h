@@ -55,7 +55,10 @@ int cfdart_request_setup(const cfdart_request* r, cfdart_desc* d, cfdart_disc_pa
 typedef struct cfdart_serve_opts {
     size_t mem_budget;       /* bytes of grid state admitted at once (two buffers per grid) */
     int capacity;            /* maximum number of live grids */
-    int chunk_steps;         /* batch steps between scheduling decisions; must divide frame_every */
+    int chunk_steps;         /* batch steps between scheduling decisions; must divide frame_every.
+                                An even count admits every grid at an even batch step, so all live
+                                grids sweep the same axis first and each step launches only the
+                                sweeps with work (odd counts give the same results, more slowly) */
     int frame_every;         /* steps between frames of one grid; 0 = no frames */
     size_t frame_host_bytes; /* pinned host frame ring; frames overwrite it cyclically */
     int realtime;            /* 1: a request is visible from its arrival time on; 0: all at once */

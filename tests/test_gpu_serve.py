@@ -84,6 +84,22 @@ def test_served_equals_alone(gpu):
     assert cfdart.state_checksum(s.grid) == rb[0].checksum
 
 
+def test_served_odd_chunks(gpu):
+    """Chunks of 5 steps admit requests at odd batch steps, so the live grids no longer share a
+    step parity and the batch launches both axes in both phases: every result is still bitwise
+    its one-at-a-time run."""
+    reqs = _mixed_requests()
+    print(f"case serve {len(reqs)} requests, chunks of 5 steps", flush=True)
+    # a live-cell cap staggers the admissions over the run (at multiples of 5 steps)
+    o = serve.opts(capacity=64, chunk_steps=5, frame_every=0, realtime=False, live_cells=3 * 160 * 160)
+    rb, sb = serve.serve_batched(reqs, o)
+    ro, _ = serve.serve_one_at_a_time(reqs, o)
+    assert sb.max_live > 1
+    for q, a, b in zip(reqs, rb, ro):
+        assert a.status == 0 and b.status == 0 and a.steps_done == b.steps_done == q.steps
+        assert a.checksum == b.checksum, (q.model, q.n, q.steps)
+
+
 def test_admission_waits_for_memory(gpu):
     reqs = _mixed_requests()
     # room for the grid state of about three of these requests at a time
