@@ -187,10 +187,11 @@ void destroy_graphs(cfdart_batch* b)
 
 // The work-lists of the live grids, per model and axis. x tiles go largest first (longest
 // processing time first: decreasing rows per wavefront, grids in slot order within a shape), which
-// shortens the tail of the launch; y tiles are all alike and go in slot order, each grid's column
-// blocks in the outer loop so that tiles sharing halo rows are consecutive. The order is built in
-// one pass per distinct tile shape (no comparison sort: a batch rebuilds its lists at every
-// admission and removal).
+// shortens the tail of the launch, each grid's row bands in the outer loop so that tiles sharing
+// halo columns are consecutive; y tiles are all alike and go in slot order, each grid's column
+// blocks in the outer loop so that tiles sharing halo rows are consecutive. A list of alike items
+// goes to the XCDs in contiguous runs (batch_item). The order is built in one pass per distinct
+// tile shape (no comparison sort: a batch rebuilds its lists at every admission and removal).
 int rebuild_tiles(cfdart_batch* b)
 {
     std::vector<Tile> v[2][2];
@@ -211,9 +212,10 @@ int rebuild_tiles(cfdart_batch* b)
                 if (!b->live[s]) continue;
                 const BatchGrid& g = b->h_grids[s];
                 if ((g.model == cfd::ALLAIRE ? 1 : 0) != m || g.xrows != xr) continue;
+                // row bands in the outer loop: tiles sharing halo columns are consecutive
                 const int ntx = (g.nx + cfd::XTILE - 1) / cfd::XTILE, nry = (g.ny + 4 * g.xrows - 1) / (4 * g.xrows);
-                for (int a = 0; a < ntx; ++a)
-                    for (int r = 0; r < nry; ++r) v[m][0].push_back(Tile{s, 0, a, r});
+                for (int r = 0; r < nry; ++r)
+                    for (int a = 0; a < ntx; ++a) v[m][0].push_back(Tile{s, 0, a, r});
             }
     }
     b->ntiles = 0;
@@ -247,6 +249,7 @@ int rebuild_tiles(cfdart_batch* b)
             }
             b->ntile[m][ax] = (int)l.size();
             lengths.ntiles[m][ax] = (int)l.size();
+            lengths.xcd[m][ax] = ax == 1 || shapes[m].size() <= 1;  // y tiles are all alike
             b->ntiles += (int)l.size();
         }
     hipLaunchKernelGGL(cfd::batch_set_lengths_kernel, dim3(1), dim3(64), 0, b->st, b->d_q, lengths);

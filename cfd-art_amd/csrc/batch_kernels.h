@@ -33,7 +33,17 @@ struct Tile {
 // list's capacity and workgroups past its length exit at once).
 struct BatchQueue {
     int ntiles[2][2];  // [model][axis]
+    int xcd[2][2];     // 1: the list's items are alike, so they may go to the XCDs in contiguous runs
 };
+
+// The work-list item of workgroup b of a launch over a list of n items, or -1 (the launch covers
+// the list's capacity). Items that are all alike go to the 8 XCDs in contiguous runs (xcd_order), so
+// that tiles sharing halo lines meet in one L2; otherwise in list order (heaviest items first).
+__device__ __forceinline__ int batch_item(int b, int n, int xcd)
+{
+    if (b >= n) return -1;
+    return xcd ? xcd_order(b, n) : b;
+}
 
 __device__ __forceinline__ GridView batch_view(BatchGrid* bg, int phase)
 {
@@ -67,8 +77,8 @@ template <int M, bool LAST, bool F>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M == ALLAIRE ? 3 : (LAST ? 4 : 1)))) void
 batch_x_kernel(BatchGrid* grids, const Tile* tiles, const BatchQueue* q, int phase)
 {
-    const int t = blockIdx.x;
-    if (t >= q->ntiles[M][0]) return;
+    const int t = batch_item(blockIdx.x, q->ntiles[M][0], q->xcd[M][0]);
+    if (t < 0) return;
     const Tile tl = tiles[t];
     BatchGrid* bg = grids + tl.grid;
     if (batch_axis(bg, phase) != 0) return;
@@ -86,8 +96,8 @@ __global__ __launch_bounds__(256) void batch_y_kernel(BatchGrid* grids, const Ti
                                                       int phase)
 {
     __shared__ double sU[Model<M>::NC * 64 * (BATCH_YT_COLS + 1)];
-    const int t = blockIdx.x;
-    if (t >= q->ntiles[M][1]) return;
+    const int t = batch_item(blockIdx.x, q->ntiles[M][1], q->xcd[M][1]);
+    if (t < 0) return;
     const Tile tl = tiles[t];
     BatchGrid* bg = grids + tl.grid;
     if (batch_axis(bg, phase) != 1) return;
