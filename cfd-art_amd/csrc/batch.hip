@@ -118,6 +118,7 @@ struct cfdart_batch {
     hipStream_t st, copy_st;
     hipGraphExec_t exec[cfd::STEP_SHAPES];  // the captured step, per shape
     bool have_graph;
+    int graph_models;           // bit m: the captured steps launch model m's sweeps
     long long gstep;            // batch steps launched since creation
     std::vector<int> cls;       // per slot: 0 / 1 alternating grid admitted at an even / odd step, 2 naive
     int nclass[3];              // live grids per class
@@ -263,9 +264,15 @@ int rebuild_tiles(cfdart_batch* b)
 // per-grid loop bookkeeping, on the batch stream; captured once per step shape (cfd::StepShape).
 // The two models' launches are not run concurrently on two streams: sharing the CUs, the batched
 // workload's step took 3 % longer than with the launches one after another.
+int models_present(const cfdart_batch* b)
+{
+    return (b->ntile[0][0] + b->ntile[0][1] > 0 ? 1 : 0) | (b->ntile[1][0] + b->ntile[1][1] > 0 ? 2 : 0);
+}
+
 int capture_graph(cfdart_batch* b)
 {
     destroy_graphs(b);
+    b->graph_models = models_present(b);
     for (int m = 0; m < 2; ++m)
         for (int ax = 0; ax < 2; ++ax)
             if (!b->d_tiles[m][ax]) {
@@ -277,12 +284,16 @@ int capture_graph(cfdart_batch* b)
     for (int k = 0; k < cfd::STEP_SHAPES; ++k) {
         hipGraph_t graph;
         BCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+        // a model with no grid in the batch gets no launches (a grid of it joining re-captures)
+        const bool euler = (b->graph_models & 1) != 0, allaire = (b->graph_models & 2) != 0;
         if (b->arith == CFDART_ARITH_FAST) {
-            cfd::launch_batch_phases_fast(EULER, b->d_grids, b->d_tiles[EULER], b->tile_cap[EULER], b->d_q, b->st, k);
-            cfd::launch_batch_phases_fast(ALLAIRE, b->d_grids, b->d_tiles[ALLAIRE], b->tile_cap[ALLAIRE], b->d_q, b->st, k);
+            if (euler) cfd::launch_batch_phases_fast(EULER, b->d_grids, b->d_tiles[EULER], b->tile_cap[EULER], b->d_q, b->st, k);
+            if (allaire)
+                cfd::launch_batch_phases_fast(ALLAIRE, b->d_grids, b->d_tiles[ALLAIRE], b->tile_cap[ALLAIRE], b->d_q, b->st, k);
         } else {
-            cfd::launch_batch_phases<EULER, false>(b->d_grids, b->d_tiles[EULER], b->tile_cap[EULER], b->d_q, b->st, k);
-            cfd::launch_batch_phases<ALLAIRE, false>(b->d_grids, b->d_tiles[ALLAIRE], b->tile_cap[ALLAIRE], b->d_q, b->st, k);
+            if (euler) cfd::launch_batch_phases<EULER, false>(b->d_grids, b->d_tiles[EULER], b->tile_cap[EULER], b->d_q, b->st, k);
+            if (allaire)
+                cfd::launch_batch_phases<ALLAIRE, false>(b->d_grids, b->d_tiles[ALLAIRE], b->tile_cap[ALLAIRE], b->d_q, b->st, k);
         }
         hipLaunchKernelGGL(cfd::batch_finalize_kernel, dim3(b->cap), dim3(64), 0, b->st, b->d_grids, b->d_status);
         BCHK(hipStreamEndCapture(b->st, &graph));
@@ -339,6 +350,7 @@ int cfdart_batch_create(int capacity, size_t mem_budget_bytes, cfdart_batch** ou
             b->ntile[m][ax] = 0;
         }
     b->have_graph = false;
+    b->graph_models = 0;
     b->gstep = 0;
     b->cls.assign(capacity, 0);
     for (int k = 0; k < 3; ++k) b->nclass[k] = 0;
@@ -649,6 +661,10 @@ int cfdart_batch_step(cfdart_batch* b, int nsteps)
         b->tiles_dirty = false;
     }
     if (b->ntiles == 0) return 0;
+    if (b->have_graph && b->graph_models != models_present(b)) {  // a model joined or left the batch
+        BCHK(hipStreamSynchronize(b->st));
+        destroy_graphs(b);
+    }
     if (!b->have_graph) {
         int rc = capture_graph(b);
         if (rc) return rc;
