@@ -233,3 +233,22 @@ def test_step_shapes_follow_each_grid_parity(gpu):
         assert nb == n == steps and tb == t and not act
         assert bitwise_equal(interior(b.download(s)), interior(ref))
     b.close()
+
+
+def test_naive_and_alternating_grids_together(gpu):
+    """Naive grids sweep x first on every step, alternating ones on every other: a batch holding
+    both launches only x-first steps on even steps and both axes on odd ones; every grid still ends
+    bitwise where it ends alone."""
+    cases = mixed_cases(5, 4)
+    b = cfdart.Batch(4)
+    splits = [cfdart.ALTERNATING, cfdart.NAIVE, cfdart.ALTERNATING, cfdart.NAIVE]
+    slots = [b.add(d, h, splitting=sp, max_steps=steps, cfl=cfl) for (d, h, cfl, steps), sp in zip(cases, splits)]
+    b.step(30)
+    b.sync()
+    for (d, h, cfl, steps), sp, s in zip(cases, splits, slots):
+        ref, t, n = alone(d, h, cfl, steps, sp)
+        tb, nb, act = b.status(s)
+        print(f"case naive+alternating slot={s} splitting={sp} {d.nx}x{d.ny} steps={steps}", flush=True)
+        assert nb == n == steps and tb == t and not act
+        assert bitwise_equal(interior(b.download(s)), interior(ref))
+    b.close()
