@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <list>
 #include <map>
@@ -83,6 +84,32 @@ __global__ void batch_admit_kernel(BatchGrid* bg, int* status)
     status[1] = c->active ? 1 : 0;
 }
 
+// A work-list segment: the tiles of one grid on one axis, in the order the sweeps take them
+// (x: row band by row band, column tiles inner; y: column block by column block, row blocks
+// inner), written at [offset, offset + n_outer * n_inner) of its model's list for that axis.
+struct TileSeg {
+    int grid, model, axis, n_outer, n_inner, offset;
+};
+struct TileLists {
+    Tile* l[2][2];  // [model][axis]
+};
+
+// Writes the tiles of every segment, one workgroup per segment: the host sends one record per
+// grid and axis rather than the tiles themselves.
+__global__ __launch_bounds__(256) void batch_expand_tiles_kernel(const TileSeg* segs, TileLists lists)
+{
+    const TileSeg sg = segs[blockIdx.x];
+    Tile* const base = sg.model == 0 ? (sg.axis == 0 ? lists.l[0][0] : lists.l[0][1])
+                                     : (sg.axis == 0 ? lists.l[1][0] : lists.l[1][1]);
+    Tile* out = base + sg.offset;
+    const int n = sg.n_outer * sg.n_inner;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int o = i / sg.n_inner, in = i - o * sg.n_inner;
+        // x: (column tile, row band) = (inner, outer); y: (column block, row block) = (outer, inner)
+        out[i] = sg.axis == 0 ? Tile{sg.grid, 0, in, o} : Tile{sg.grid, 1, o, in};
+    }
+}
+
 }  // namespace cfd
 
 using cfd::BatchGrid;
@@ -133,9 +160,11 @@ struct cfdart_batch {
     // the stream has passed an entry's copies: the admission returns without waiting
     std::vector<AdmitStage> stages;
     size_t stage_next;
-    // pinned host staging of the tile work-lists, and the event after their last upload
-    Tile* h_tiles[2][2];
-    size_t h_tile_cap[2][2];
+    // the work-lists' segments (one per live grid and axis): pinned host staging, its device
+    // copy, their capacity, and the event after their last upload and expansion
+    cfd::TileSeg* h_segs;
+    cfd::TileSeg* d_segs;
+    size_t seg_cap;
     hipEvent_t tiles_ev;
     bool tiles_ev_pending;
     // frame staging ring in device memory: regions still being copied to the host
@@ -191,20 +220,46 @@ void destroy_graphs(cfdart_batch* b)
 // shortens the tail of the launch, each grid's row bands in the outer loop so that tiles sharing
 // halo columns are consecutive; y tiles are all alike and go in slot order, each grid's column
 // blocks in the outer loop so that tiles sharing halo rows are consecutive. A list of alike items
-// goes to the XCDs in contiguous runs (batch_item). The order is built in one pass per distinct
-// tile shape (no comparison sort: a batch rebuilds its lists at every admission and removal).
+// goes to the XCDs in contiguous runs (batch_item). The host lays out one segment per grid and
+// axis (no comparison sort: a batch rebuilds its lists at every admission and removal) and a
+// kernel writes the tiles, so a rebuild costs the host O(grids), not O(tiles).
 int rebuild_tiles(cfdart_batch* b)
 {
-    std::vector<Tile> v[2][2];
     std::vector<int> shapes[2];  // distinct x rows per wavefront, per model
+    size_t nseg = 0;
     for (int s = 0; s < b->cap; ++s) {
         if (!b->live[s]) continue;
         const BatchGrid& g = b->h_grids[s];
         const int m = g.model == cfd::ALLAIRE ? 1 : 0;
         if (std::find(shapes[m].begin(), shapes[m].end(), g.xrows) == shapes[m].end()) shapes[m].push_back(g.xrows);
+        nseg += 2;
+    }
+    // the pinned staging of the previous upload is rewritten only once that upload (and the
+    // expansion that read its device copy) has finished
+    if (b->tiles_ev_pending) {
+        BCHK(hipEventSynchronize(b->tiles_ev));
+        b->tiles_ev_pending = false;
+    }
+    if (nseg > b->seg_cap) {
+        if (b->h_segs) BCHK(hipHostFree(b->h_segs));
+        if (b->d_segs) BCHK(hipFree(b->d_segs));
+        b->h_segs = nullptr;
+        b->d_segs = nullptr;
+        b->seg_cap = 0;
+        const size_t cap = std::max<size_t>(nseg, 256);
+        BCHK(hipHostMalloc((void**)&b->h_segs, sizeof(cfd::TileSeg) * cap, hipHostMallocDefault));
+        BCHK(hipMalloc(&b->d_segs, sizeof(cfd::TileSeg) * cap));
+        b->seg_cap = cap;
+    }
+    size_t len[2][2] = {{0, 0}, {0, 0}};
+    size_t k = 0;
+    for (int s = 0; s < b->cap; ++s) {
+        if (!b->live[s]) continue;
+        const BatchGrid& g = b->h_grids[s];
+        const int m = g.model == cfd::ALLAIRE ? 1 : 0;
         const int nbx = (g.nx + cfd::BATCH_YT_COLS - 1) / cfd::BATCH_YT_COLS, nby = (g.ny + cfd::YT_ROWS - 1) / cfd::YT_ROWS;
-        for (int a = 0; a < nbx; ++a)
-            for (int r = 0; r < nby; ++r) v[m][1].push_back(Tile{s, 1, a, r});
+        b->h_segs[k++] = cfd::TileSeg{s, m, 1, nbx, nby, (int)len[m][1]};
+        len[m][1] += (size_t)nbx * nby;
     }
     for (int m = 0; m < 2; ++m) {
         std::sort(shapes[m].begin(), shapes[m].end(), [](int x, int y) { return x > y; });
@@ -213,46 +268,39 @@ int rebuild_tiles(cfdart_batch* b)
                 if (!b->live[s]) continue;
                 const BatchGrid& g = b->h_grids[s];
                 if ((g.model == cfd::ALLAIRE ? 1 : 0) != m || g.xrows != xr) continue;
-                // row bands in the outer loop: tiles sharing halo columns are consecutive
                 const int ntx = (g.nx + cfd::XTILE - 1) / cfd::XTILE, nry = (g.ny + 4 * g.xrows - 1) / (4 * g.xrows);
-                for (int r = 0; r < nry; ++r)
-                    for (int a = 0; a < ntx; ++a) v[m][0].push_back(Tile{s, 0, a, r});
+                b->h_segs[k++] = cfd::TileSeg{s, m, 0, nry, ntx, (int)len[m][0]};
+                len[m][0] += (size_t)nry * ntx;
             }
     }
+    if (k != nseg) return bfail(CFDART_E_HIP, "work-list segments miscounted");
+    for (int m = 0; m < 2; ++m)
+        for (int ax = 0; ax < 2; ++ax)
+            if (len[m][ax] > (size_t)INT32_MAX) return bfail(CFDART_E_NOMEM, "work-list longer than 2^31 tiles");
     b->ntiles = 0;
     BatchQueue lengths{};
-    // the pinned staging of the previous upload is rewritten only once that upload has finished
-    if (b->tiles_ev_pending) {
-        BCHK(hipEventSynchronize(b->tiles_ev));
-        b->tiles_ev_pending = false;
-    }
+    cfd::TileLists lists;
     for (int m = 0; m < 2; ++m)
         for (int ax = 0; ax < 2; ++ax) {
-            const std::vector<Tile>& l = v[m][ax];
-            if (l.size() > b->tile_cap[m][ax]) {
+            if (len[m][ax] > b->tile_cap[m][ax]) {
                 BCHK(hipStreamSynchronize(b->st));
                 if (b->d_tiles[m][ax]) BCHK(hipFree(b->d_tiles[m][ax]));
-                b->tile_cap[m][ax] = std::max(l.size(), 2 * b->tile_cap[m][ax]);
+                b->d_tiles[m][ax] = nullptr;
+                b->tile_cap[m][ax] = std::max(len[m][ax], 2 * b->tile_cap[m][ax]);
                 BCHK(hipMalloc(&b->d_tiles[m][ax], sizeof(Tile) * b->tile_cap[m][ax]));
                 destroy_graphs(b);  // re-captured with the new list and launch size
             }
-            if (l.size() > b->h_tile_cap[m][ax]) {
-                if (b->h_tiles[m][ax]) BCHK(hipHostFree(b->h_tiles[m][ax]));
-                b->h_tiles[m][ax] = nullptr;
-                b->h_tile_cap[m][ax] = 0;
-                BCHK(hipHostMalloc((void**)&b->h_tiles[m][ax], sizeof(Tile) * b->tile_cap[m][ax], hipHostMallocDefault));
-                b->h_tile_cap[m][ax] = b->tile_cap[m][ax];
-            }
-            if (!l.empty()) {
-                std::memcpy(b->h_tiles[m][ax], l.data(), sizeof(Tile) * l.size());
-                BCHK(hipMemcpyAsync(b->d_tiles[m][ax], b->h_tiles[m][ax], sizeof(Tile) * l.size(),
-                                    hipMemcpyHostToDevice, b->st));
-            }
-            b->ntile[m][ax] = (int)l.size();
-            lengths.ntiles[m][ax] = (int)l.size();
+            lists.l[m][ax] = b->d_tiles[m][ax];
+            b->ntile[m][ax] = (int)len[m][ax];
+            lengths.ntiles[m][ax] = (int)len[m][ax];
             lengths.xcd[m][ax] = ax == 1 || shapes[m].size() <= 1;  // y tiles are all alike
-            b->ntiles += (int)l.size();
+            b->ntiles += (int)len[m][ax];
         }
+    if (nseg > 0) {
+        BCHK(hipMemcpyAsync(b->d_segs, b->h_segs, sizeof(cfd::TileSeg) * nseg, hipMemcpyHostToDevice, b->st));
+        hipLaunchKernelGGL(cfd::batch_expand_tiles_kernel, dim3((unsigned)nseg), dim3(256), 0, b->st, b->d_segs, lists);
+        BCHK(hipGetLastError());
+    }
     hipLaunchKernelGGL(cfd::batch_set_lengths_kernel, dim3(1), dim3(64), 0, b->st, b->d_q, lengths);
     BCHK(hipGetLastError());
     BCHK(hipEventRecord(b->tiles_ev, b->st));
@@ -363,11 +411,9 @@ int cfdart_batch_create(int capacity, size_t mem_budget_bytes, cfdart_batch** ou
     b->d_ring = nullptr;
     b->ring_cap = b->ring_head = 0;
     b->stage_next = 0;
-    for (int m = 0; m < 2; ++m)
-        for (int ax = 0; ax < 2; ++ax) {
-            b->h_tiles[m][ax] = nullptr;
-            b->h_tile_cap[m][ax] = 0;
-        }
+    b->h_segs = nullptr;
+    b->d_segs = nullptr;
+    b->seg_cap = 0;
     b->tiles_ev_pending = false;
     b->tiles_ev = nullptr;
     if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess ||
@@ -439,9 +485,8 @@ void cfdart_batch_destroy(cfdart_batch* b)
         (void)hipEventDestroy(a.done);
         (void)hipHostFree(a.rec);
     }
-    for (int m = 0; m < 2; ++m)
-        for (int ax = 0; ax < 2; ++ax)
-            if (b->h_tiles[m][ax]) (void)hipHostFree(b->h_tiles[m][ax]);
+    if (b->h_segs) (void)hipHostFree(b->h_segs);
+    if (b->d_segs) (void)hipFree(b->d_segs);
     if (b->tiles_ev) (void)hipEventDestroy(b->tiles_ev);
     (void)hipStreamDestroy(b->st);
     (void)hipStreamDestroy(b->copy_st);

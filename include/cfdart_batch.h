@@ -9,7 +9,8 @@
  * every live grid from device work-lists (one tile per workgroup; a grid's tiles for the axis it
  * does not sweep in that phase exit at once), and the step (two phases, loop bookkeeping) is
  * captured once in a HIP graph and replayed. Grids join and leave between steps by editing
- * device-resident tables; the graph is re-captured only when a work-list outgrows its capacity.
+ * device-resident tables; the graph is re-captured only when a work-list outgrows its capacity or
+ * the first grid of a model joins (or the last one leaves) the batch.
  *
  * Splitting schemes: CFDART_NAIVE and CFDART_ALTERNATING (two sweeps per step).
  * Admissions, removals and tile changes take effect at the next cfdart_batch_step, which
