@@ -96,3 +96,84 @@ def test_mass_history_matches_compute_dt(gpu, model):
         ora.n += 1
     if model == 0:  # closed box: mass conserved to round-off
         assert np.max(np.abs(hist[:, 1] - hist[0, 1])) <= 1e-13 * hist[0, 1]
+
+
+def _frame_state(planes, d):
+    """A host state (cfdart.h layout) whose interior is a frame's planes ([component][y][x])."""
+    s = np.zeros((d.ny + 2 * d.ng, d.nx + 2 * d.ng, planes.shape[0]))
+    s[d.ng:-d.ng, d.ng:-d.ng, :] = np.moveaxis(planes, 0, 2)
+    return s
+
+
+def test_batch_frames_match_state(gpu):
+    """Batch frames queued between steps without waiting (staged in the device ring, copied to
+    the host on the copy stream) hold each grid's state at their point of the stream: a frame's
+    checksum equals the device checksum queued right after it. The small grids' frames wrap the
+    128 MiB staging ring while earlier copies are still in flight; a large grid joining later
+    regrows the ring. The last frame of each grid equals its downloaded state."""
+    L = cfdart.lib()
+    specs = [(0, 300, 300), (1, 257, 200), (0, 130, 130), (1, 64, 64)]
+    b = cfdart.Batch(8)
+
+    def make(model, nx, ny, seed):
+        if model == 0:
+            d, h = cfdart.configs.euler_discs(nx, 4, (0.05, 0.15), (0, 0, 0, 0), seed=seed, ny=ny)
+            return d, h, 0.8
+        d, h = cfdart.configs.allaire_bubbles(nx, 3, seed=seed, ny=ny)
+        return d, h, 0.5
+
+    def nbytes(d):
+        return 8 * d.nx * d.ny * (6 if d.model == 1 else 4)
+
+    grids = []
+    for k, (model, nx, ny) in enumerate(specs):
+        d, h, cfl = make(model, nx, ny, 40 + k)
+        grids.append((b.add(d, h, max_steps=100, cfl=cfl), d))
+    dl, hl, cl = make(1, 1190, 1190, 50)
+    rounds1, rounds2 = 24, 3
+    small = sum(nbytes(d) for _, d in grids)
+    total = rounds1 * small + (rounds2 + 1) * (small + nbytes(dl))
+    nframes = rounds1 * len(grids) + (rounds2 + 1) * (len(grids) + 1)
+    assert rounds1 * small > 128 * 2**20  # the first phase wraps the staging ring
+    host = L.cfdart_host_alloc(total)
+    sums = L.cfdart_host_alloc(8 * nframes)
+    assert host and sums
+    taken = []  # (slot, desc, byte offset, frame index)
+    try:
+        off = 0
+
+        def frame_all():
+            nonlocal off
+            for s, d in grids:
+                i = len(taken)
+                b.frame(s, host + off)
+                cfdart._lib._check(L.cfdart_batch_checksum_async(b.h, s, C.c_void_p(sums + 8 * i)))
+                taken.append((s, d, off, i))
+                off += nbytes(d)
+
+        for r in range(rounds1):
+            frame_all()
+            b.step(1)
+        grids.append((b.add(dl, hl, max_steps=100, cfl=cl), dl))
+        for r in range(rounds2):
+            frame_all()
+            b.step(1)
+        frame_all()  # the final states, compared with downloads below
+        print(f"case batch frames: {len(taken)} frames, {off / 2**20:.0f} MiB", flush=True)
+        b.frames_done()
+        b.sync()
+        last = {}
+        for s, d, o, i in taken:
+            nc = 6 if d.model == 1 else 4
+            planes = np.ctypeslib.as_array((C.c_double * (nc * d.ny * d.nx)).from_address(host + o))
+            planes = planes.reshape(nc, d.ny, d.nx).copy()
+            want = C.c_ulonglong.from_address(sums + 8 * i).value
+            assert cfdart.state_checksum(_frame_state(planes, d)) == want, (s, i)
+            last[s] = planes
+        for s, d in grids:
+            st = b.download(s)
+            assert np.array_equal(last[s].view(np.uint64), np.moveaxis(st[d.ng:-d.ng, d.ng:-d.ng, :], 2, 0).view(np.uint64))
+    finally:
+        L.cfdart_host_free(C.c_void_p(host))
+        L.cfdart_host_free(C.c_void_p(sums))
+    b.close()
