@@ -125,6 +125,8 @@ def lib():
         L.cfdart_batch_set_arith.argtypes = [C.c_void_p, C.c_int]
         L.cfdart_batch_info.argtypes = [C.c_void_p, P(C.c_int), P(C.c_size_t), P(C.c_int)]
         L.cfdart_batch_frame.argtypes = [C.c_void_p, C.c_int, dp]
+        L.cfdart_batch_frames.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_void_p),
+                                          C.POINTER(C.c_int)]
         L.cfdart_batch_frames_done.argtypes = [C.c_void_p]
         L.cfdart_batch_stream.argtypes = [C.c_void_p]
         L.cfdart_batch_stream.restype = C.c_void_p
@@ -433,6 +435,16 @@ class Batch:
 
     def frame(self, slot, host_ptr):
         _check(lib().cfdart_batch_frame(self.h, slot, C.cast(host_ptr, C.POINTER(C.c_double))))
+
+    def frames(self, slots, host_ptrs):
+        """Frames of several grids in one group; returns each frame's status (0 taken,
+        -2 skipped: the staging ring could not grow within the memory budget)."""
+        n = len(slots)
+        s = (C.c_int * n)(*slots)
+        p = (C.c_void_p * n)(*host_ptrs)
+        st = (C.c_int * n)()
+        _check(lib().cfdart_batch_frames(self.h, n, s, p, st))
+        return list(st)
 
     def frames_done(self):
         _check(lib().cfdart_batch_frames_done(self.h))

@@ -17,7 +17,9 @@
 #include <cstring>
 #include <list>
 #include <map>
+#include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "batch_kernels.h"
@@ -124,9 +126,16 @@ struct AdmitStage {
 };
 constexpr int ADMIT_STAGES = 32;
 
+// An event shared by the frames of one group (destroyed with its last holder).
+using EventRef = std::shared_ptr<std::remove_pointer<hipEvent_t>::type>;
+inline EventRef event_ref(hipEvent_t e)
+{
+    return EventRef(e, [](hipEvent_t x) { (void)hipEventDestroy(x); });
+}
+
 struct FrameRegion {
     size_t off, len;  // doubles
-    hipEvent_t done;  // the device-to-host copy out of the region has finished
+    EventRef done;    // the device-to-host copies out of the region's group have finished
 };
 
 struct cfdart_batch {
@@ -153,6 +162,7 @@ struct cfdart_batch {
     int ntiles;
     std::multimap<size_t, double*> pool;  // freed state buffers for reuse, by size
     int* d_status;           // per slot: steps taken, flags (1 running, 2 error)
+    int* h_status;           // its pinned host copy (cfdart_batch_poll)
     bool tiles_dirty;        // the work-list is rebuilt before the next step
     void* d_draws;           // initial-condition draws of the admission in progress
     unsigned long long* d_sum;
@@ -405,6 +415,7 @@ int cfdart_batch_create(int capacity, size_t mem_budget_bytes, cfdart_batch** ou
     b->arith = CFDART_ARITH_EXACT;
     b->ntiles = 0;
     b->d_status = nullptr;
+    b->h_status = nullptr;
     b->tiles_dirty = false;
     b->d_draws = nullptr;
     b->d_sum = nullptr;
@@ -434,9 +445,11 @@ int cfdart_batch_create(int capacity, size_t mem_budget_bytes, cfdart_batch** ou
         a.draws = (char*)p + sizeof(BatchGrid);
         b->stages.push_back(a);
     }
-    if (b->stages.size() != (size_t)ADMIT_STAGES) {
+    if (hipHostMalloc((void**)&b->h_status, sizeof(int) * 2 * (size_t)capacity, hipHostMallocDefault) != hipSuccess)
+        b->h_status = nullptr;
+    if (b->stages.size() != (size_t)ADMIT_STAGES || !b->h_status) {
         cfdart_batch_destroy(b);
-        return bfail(CFDART_E_NOMEM, "pinned admission staging");
+        return bfail(CFDART_E_NOMEM, "pinned host staging");
     }
     if (hipMalloc(&b->d_grids, sizeof(BatchGrid) * capacity) != hipSuccess ||
         hipMalloc(&b->d_q, sizeof(BatchQueue)) != hipSuccess ||
@@ -480,11 +493,12 @@ void cfdart_batch_destroy(cfdart_batch* b)
     if (b->d_draws) (void)hipFree(b->d_draws);
     if (b->d_sum) (void)hipFree(b->d_sum);
     if (b->d_ring) (void)hipFree(b->d_ring);
-    for (FrameRegion& r : b->inflight) (void)hipEventDestroy(r.done);
+    b->inflight.clear();
     for (AdmitStage& a : b->stages) {
         (void)hipEventDestroy(a.done);
         (void)hipHostFree(a.rec);
     }
+    if (b->h_status) (void)hipHostFree(b->h_status);
     if (b->h_segs) (void)hipHostFree(b->h_segs);
     if (b->d_segs) (void)hipFree(b->d_segs);
     if (b->tiles_ev) (void)hipEventDestroy(b->tiles_ev);
@@ -830,68 +844,104 @@ int cfdart_batch_info(cfdart_batch* b, int* live, size_t* bytes, int* tiles)
     return 0;
 }
 
+int cfdart_batch_frames(cfdart_batch* b, int n, const int* slots, double* const* host_dsts, int* status)
+{
+    if (!b || n < 0 || (n > 0 && (!slots || !host_dsts))) return bfail(CFDART_E_ARG, "bad frame request");
+    for (int i = 0; i < n; ++i) {
+        int rc = check_slot(b, slots[i]);
+        if (rc) return rc;
+        if (!host_dsts[i]) return bfail(CFDART_E_ARG, "null frame buffer");
+    }
+    // frames staged in the ring whose copies to the host are not queued yet
+    struct Pending {
+        size_t off, len;
+        double* dst;
+    };
+    std::vector<Pending> pend;
+    // queues the copies of the staged frames to the host behind one event on the batch stream,
+    // and one event after them on the copy stream, shared by their ring regions
+    auto flush = [&]() -> int {
+        if (pend.empty()) return 0;
+        hipEvent_t ready, done;
+        BCHK(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+        EventRef hold_ready = event_ref(ready);
+        BCHK(hipEventRecord(ready, b->st));
+        BCHK(hipStreamWaitEvent(b->copy_st, ready, 0));
+        for (const Pending& p : pend)
+            BCHK(hipMemcpyAsync(p.dst, b->d_ring + p.off, sizeof(double) * p.len, hipMemcpyDeviceToHost, b->copy_st));
+        BCHK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+        EventRef hold_done = event_ref(done);
+        BCHK(hipEventRecord(done, b->copy_st));
+        for (const Pending& p : pend) b->inflight.push_back(FrameRegion{p.off, p.len, hold_done});
+        pend.clear();
+        return 0;
+    };
+    for (int i = 0; i < n; ++i) {
+        if (status) status[i] = 0;
+        const BatchGrid& g = b->h_grids[slots[i]];
+        const int nc = g.model == cfd::ALLAIRE ? 6 : 4;
+        const size_t len = (size_t)g.nx * g.ny * nc;
+        if (len > b->ring_cap / 2) {
+            // grow the ring to hold four frames of this size (at least 16 Mi doubles, 128 MiB); its
+            // bytes count against the batch's memory budget like the grids' states
+            const size_t cap = std::max<size_t>(4 * len, size_t(1) << 24);
+            const size_t used_without = b->used - sizeof(double) * b->ring_cap;
+            if (b->budget && used_without + sizeof(double) * cap > b->budget) {
+                if (status) status[i] = CFDART_E_NOMEM;
+                if (n == 1) return bfail(CFDART_E_NOMEM, "frame staging ring exceeds the memory budget");
+                continue;  // this frame is skipped; the others are taken
+            }
+            int rc = flush();
+            if (rc) return rc;
+            BCHK(hipStreamSynchronize(b->st));
+            BCHK(hipStreamSynchronize(b->copy_st));
+            b->inflight.clear();
+            if (b->d_ring) BCHK(hipFree(b->d_ring));
+            b->d_ring = nullptr;
+            b->used = used_without;
+            b->ring_cap = 0;
+            b->ring_head = 0;
+            if (hipMalloc(&b->d_ring, sizeof(double) * cap) != hipSuccess) return bfail(CFDART_E_NOMEM, "frame staging ring");
+            b->ring_cap = cap;
+            b->used += sizeof(double) * cap;
+        }
+        size_t off = b->ring_head;
+        if (off + len > b->ring_cap) off = 0;
+        // a region staged earlier in this call has its copy queued before the region is reused
+        for (const Pending& p : pend)
+            if (p.off < off + len && off < p.off + p.len) {
+                int rc = flush();
+                if (rc) return rc;
+                break;
+            }
+        // the region must not hold a frame whose copy to the host is still running
+        for (auto it = b->inflight.begin(); it != b->inflight.end();) {
+            if (it->off < off + len && off < it->off + it->len) {
+                BCHK(hipStreamWaitEvent(b->st, it->done.get(), 0));
+                it = b->inflight.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        // stage the interior planes (contiguous, pitch removed) on the batch stream: the grid's
+        // state after a whole step is buf[0]
+        BCHK(hipMemcpy2DAsync(b->d_ring + off, sizeof(double) * g.nx, g.buf[0], sizeof(double) * g.pitch,
+                              sizeof(double) * g.nx, (size_t)g.ny * nc, hipMemcpyDeviceToDevice, b->st));
+        pend.push_back(Pending{off, len, host_dsts[i]});
+        b->ring_head = off + len;
+    }
+    return flush();
+}
+
 int cfdart_batch_frame(cfdart_batch* b, int slot, double* host_dst)
 {
-    int rc = check_slot(b, slot);
-    if (rc) return rc;
-    if (!host_dst) return bfail(CFDART_E_ARG, "null frame buffer");
-    const BatchGrid& g = b->h_grids[slot];
-    const int nc = g.model == cfd::ALLAIRE ? 6 : 4;
-    const size_t n = (size_t)g.nx * g.ny * nc;
-    if (n > b->ring_cap / 2) {
-        // grow the ring to hold four frames of this size (at least 16 Mi doubles, 128 MiB); its
-        // bytes count against the batch's memory budget like the grids' states
-        const size_t cap = std::max<size_t>(4 * n, size_t(1) << 24);
-        const size_t used_without = b->used - sizeof(double) * b->ring_cap;
-        if (b->budget && used_without + sizeof(double) * cap > b->budget)
-            return bfail(CFDART_E_NOMEM, "frame staging ring exceeds the memory budget");
-        BCHK(hipStreamSynchronize(b->st));
-        BCHK(hipStreamSynchronize(b->copy_st));
-        for (FrameRegion& r : b->inflight) (void)hipEventDestroy(r.done);
-        b->inflight.clear();
-        if (b->d_ring) BCHK(hipFree(b->d_ring));
-        b->d_ring = nullptr;
-        b->used = used_without;
-        b->ring_cap = 0;
-        b->ring_head = 0;
-        if (hipMalloc(&b->d_ring, sizeof(double) * cap) != hipSuccess) return bfail(CFDART_E_NOMEM, "frame staging ring");
-        b->ring_cap = cap;
-        b->used += sizeof(double) * cap;
-    }
-    size_t off = b->ring_head;
-    if (off + n > b->ring_cap) off = 0;
-    // the region must not hold a frame whose copy to the host is still running
-    for (auto it = b->inflight.begin(); it != b->inflight.end();) {
-        if (it->off < off + n && off < it->off + it->len) {
-            BCHK(hipStreamWaitEvent(b->st, it->done, 0));
-            (void)hipEventDestroy(it->done);
-            it = b->inflight.erase(it);
-        } else {
-            ++it;
-        }
-    }
-    // stage the interior planes (contiguous, pitch removed) on the batch stream: the grid's
-    // state after a whole step is buf[0]
-    BCHK(hipMemcpy2DAsync(b->d_ring + off, sizeof(double) * g.nx, g.buf[0], sizeof(double) * g.pitch,
-                          sizeof(double) * g.nx, (size_t)g.ny * nc, hipMemcpyDeviceToDevice, b->st));
-    hipEvent_t ready, done;
-    BCHK(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
-    BCHK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
-    BCHK(hipEventRecord(ready, b->st));
-    BCHK(hipStreamWaitEvent(b->copy_st, ready, 0));
-    BCHK(hipMemcpyAsync(host_dst, b->d_ring + off, sizeof(double) * n, hipMemcpyDeviceToHost, b->copy_st));
-    BCHK(hipEventRecord(done, b->copy_st));
-    (void)hipEventDestroy(ready);
-    b->inflight.push_back(FrameRegion{off, n, done});
-    b->ring_head = off + n;
-    return 0;
+    return cfdart_batch_frames(b, 1, &slot, &host_dst, nullptr);
 }
 
 int cfdart_batch_frames_done(cfdart_batch* b)
 {
     if (!b) return bfail(CFDART_E_ARG, "null batch");
     BCHK(hipStreamSynchronize(b->copy_st));
-    for (FrameRegion& r : b->inflight) (void)hipEventDestroy(r.done);
     b->inflight.clear();
     return 0;
 }
@@ -899,12 +949,16 @@ int cfdart_batch_frames_done(cfdart_batch* b)
 int cfdart_batch_poll(cfdart_batch* b, int* steps, int* flags)
 {
     if (!b) return bfail(CFDART_E_ARG, "null batch");
-    std::vector<int> st(2 * (size_t)b->cap);
-    BCHK(hipMemcpyAsync(st.data(), b->d_status, sizeof(int) * st.size(), hipMemcpyDeviceToHost, b->st));
+    // only the slots up to the last live one are copied (into pinned memory)
+    int hi = 0;
+    for (int s = 0; s < b->cap; ++s)
+        if (b->live[s]) hi = s + 1;
+    if (hi > 0) BCHK(hipMemcpyAsync(b->h_status, b->d_status, sizeof(int) * 2 * hi, hipMemcpyDeviceToHost, b->st));
     BCHK(hipStreamSynchronize(b->st));
     for (int s = 0; s < b->cap; ++s) {
-        if (steps) steps[s] = b->live[s] ? st[2 * s] : 0;
-        if (flags) flags[s] = b->live[s] ? st[2 * s + 1] : 0;
+        const bool on = s < hi && b->live[s];
+        if (steps) steps[s] = on ? b->h_status[2 * s] : 0;
+        if (flags) flags[s] = on ? b->h_status[2 * s + 1] : 0;
     }
     return 0;
 }

@@ -55,6 +55,18 @@ struct FrameRing {
     }
 };
 
+// Pinned host array of checksums that the device writes (cfdart_batch_checksum_async).
+struct PinnedSums {
+    unsigned long long* p = nullptr;
+    explicit PinnedSums(size_t n)
+    {
+        if (n) p = (unsigned long long*)cfdart_host_alloc(sizeof(unsigned long long) * n);
+        if (p) std::memset(p, 0, sizeof(unsigned long long) * n);
+    }
+    ~PinnedSums() { cfdart_host_free(p); }
+    unsigned long long& operator[](size_t i) { return p[i]; }
+};
+
 }  // namespace
 
 extern "C" {
@@ -170,6 +182,15 @@ int cfdart_serve_batched(const cfdart_request* reqs, int n, const cfdart_serve_o
     }
     FrameRing ring(o->frame_every > 0 ? o->frame_host_bytes : 0);
     std::vector<int> owner(o->capacity, -1), steps(o->capacity), flags(o->capacity);
+    std::vector<int> fslots, fstatus;  // the frames of a chunk
+    std::vector<double*> fdsts;
+    std::vector<size_t> fbytes;
+    PinnedSums csum((size_t)n);  // the finished requests' checksums, written by the device
+    std::vector<char> summed((size_t)n, 0);
+    if (n > 0 && !csum.p) {
+        cfdart_batch_destroy(b);
+        return CFDART_E_NOMEM;
+    }
     std::deque<int> waiting;
     int next = 0, done = 0, live = 0;
     long long live_cells = 0;
@@ -236,36 +257,55 @@ int cfdart_serve_batched(const cfdart_request* reqs, int n, const cfdart_serve_o
         if ((rc = cfdart_batch_step(b, o->chunk_steps)) != 0) return fail_all(rc);
         if ((rc = cfdart_batch_poll(b, steps.data(), flags.data())) != 0) return fail_all(rc);
         sum->chunks += 1;
-        for (int s = 0; s < o->capacity; ++s) {
+        // a frame each time a grid's step count reaches a multiple of frame_every: the chunk's
+        // frames go to the batch as one group, before any grid of the chunk is retired
+        fslots.clear();
+        fdsts.clear();
+        fbytes.clear();
+        for (int s = 0; s < o->capacity && o->frame_every > 0; ++s) {
             const int r = owner[s];
             if (r < 0) continue;
             cfdart_request_result& res = out[r];
-            // a frame each time the grid's step count reaches a multiple of frame_every
-            if (o->frame_every > 0 && steps[s] / o->frame_every > res.frames && steps[s] % o->frame_every == 0) {
+            if (steps[s] / o->frame_every > res.frames && steps[s] % o->frame_every == 0) {
                 cfdart_desc d;
                 cfdart_disc_params dp;
                 cfdart_bubble_params bp;
                 double cfl;
                 cfdart_request_setup(&reqs[r], &d, &dp, &bp, &cfl);
+                const size_t fb = frame_bytes(d);
                 // a frame that finds no room (host ring smaller than the frame, or a device staging
                 // ring the memory budget cannot hold) is skipped; the grid keeps running
-                double* dst = ring.take(frame_bytes(d));
+                double* dst = ring.take(fb);
                 if (dst) {
-                    rc = cfdart_batch_frame(b, s, dst);
-                    if (rc && rc != CFDART_E_NOMEM) return fail_all(rc);
-                    if (!rc) {
-                        sum->frames += 1;
-                        sum->frame_bytes += (double)frame_bytes(d);
-                    }
-                    cfdart_batch_info(b, nullptr, &held, nullptr);
-                    sum->peak_state_bytes = std::max(sum->peak_state_bytes, (double)held);
+                    fslots.push_back(s);
+                    fdsts.push_back(dst);
+                    fbytes.push_back(fb);
                 }
                 res.frames = steps[s] / o->frame_every;
             }
+        }
+        if (!fslots.empty()) {
+            fstatus.assign(fslots.size(), 0);
+            rc = cfdart_batch_frames(b, (int)fslots.size(), fslots.data(), fdsts.data(), fstatus.data());
+            if (rc) return fail_all(rc);
+            for (size_t i = 0; i < fslots.size(); ++i)
+                if (fstatus[i] == 0) {
+                    sum->frames += 1;
+                    sum->frame_bytes += (double)fbytes[i];
+                }
+            cfdart_batch_info(b, nullptr, &held, nullptr);
+            sum->peak_state_bytes = std::max(sum->peak_state_bytes, (double)held);
+        }
+        for (int s = 0; s < o->capacity; ++s) {
+            const int r = owner[s];
+            if (r < 0) continue;
+            cfdart_request_result& res = out[r];
             if (!(flags[s] & 1)) {
                 res.steps_done = steps[s];
                 res.status = (flags[s] & 2) ? CFDART_E_UNPHYS : 0;
-                if ((rc = cfdart_batch_checksum(b, s, &res.checksum)) != 0) return fail_all(rc);
+                // the checksum lands in pinned memory once the stream has passed it (read at the end)
+                if ((rc = cfdart_batch_checksum_async(b, s, &csum[r])) != 0) return fail_all(rc);
+                summed[r] = 1;
                 res.finished = seconds_since(t0);
                 res.service = res.finished - res.admitted;
                 sum->cell_updates += (double)reqs[r].n * reqs[r].n * steps[s];
@@ -280,6 +320,8 @@ int cfdart_serve_batched(const cfdart_request* reqs, int n, const cfdart_serve_o
     }
     rc = cfdart_batch_sync(b);
     sum->makespan = seconds_since(t0);
+    for (int r = 0; r < n && rc == 0; ++r)
+        if (summed[r]) out[r].checksum = csum[r];
     cfdart_batch_destroy(b);
     return rc;
 }

@@ -96,6 +96,13 @@ int cfdart_batch_info(cfdart_batch* b, int* live, size_t* bytes, int* tiles);
  * would exceed the budget returns CFDART_E_NOMEM. Completion: cfdart_batch_sync or
  * cfdart_batch_frames_done. */
 int cfdart_batch_frame(cfdart_batch* b, int slot, double* host_dst);
+/* Frames of n grids at one point of the batch stream, as n cfdart_batch_frame calls but with
+ * one event between the streams for the group (the copies of a group are queued together; a
+ * group that would reuse its own ring space is split). status (n entries, may be NULL) receives
+ * 0 for each frame taken and CFDART_E_NOMEM for one skipped because the ring could not grow
+ * within the memory budget; the call then still takes the others and returns 0. No reference
+ * counterpart (the serving loop's frames of one chunk). */
+int cfdart_batch_frames(cfdart_batch* b, int n, const int* slots, double* const* host_dsts, int* status);
 int cfdart_batch_frames_done(cfdart_batch* b);
 void* cfdart_batch_stream(cfdart_batch* b);
 

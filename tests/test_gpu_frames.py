@@ -110,7 +110,8 @@ def test_batch_frames_match_state(gpu):
     the host on the copy stream) hold each grid's state at their point of the stream: a frame's
     checksum equals the device checksum queued right after it. The small grids' frames wrap the
     128 MiB staging ring while earlier copies are still in flight; a large grid joining later
-    regrows the ring. The last frame of each grid equals its downloaded state."""
+    regrows the ring. Frames are taken one per call and in groups (cfdart_batch_frames, one event
+    between the streams per group). The last frame of each grid equals its downloaded state."""
     L = cfdart.lib()
     specs = [(0, 300, 300), (1, 257, 200), (0, 130, 130), (1, 64, 64)]
     b = cfdart.Batch(8)
@@ -142,23 +143,33 @@ def test_batch_frames_match_state(gpu):
     try:
         off = 0
 
-        def frame_all():
+        def frame_all(group):
+            # one frame per call, or the frames of every grid as one group (cfdart_batch_frames)
             nonlocal off
-            for s, d in grids:
-                i = len(taken)
-                b.frame(s, host + off)
+            first = len(taken)
+            if group:
+                ptrs = []
+                for s, d in grids:
+                    ptrs.append(host + off)
+                    taken.append((s, d, off, len(taken)))
+                    off += nbytes(d)
+                assert b.frames([s for s, _ in grids], ptrs) == [0] * len(grids)
+            for k, (s, d) in enumerate(grids):
+                if not group:
+                    b.frame(s, host + off)
+                    taken.append((s, d, off, len(taken)))
+                    off += nbytes(d)
+                i = first + k
                 cfdart._lib._check(L.cfdart_batch_checksum_async(b.h, s, C.c_void_p(sums + 8 * i)))
-                taken.append((s, d, off, i))
-                off += nbytes(d)
 
         for r in range(rounds1):
-            frame_all()
+            frame_all(r % 2 == 1)
             b.step(1)
         grids.append((b.add(dl, hl, max_steps=100, cfl=cl), dl))
         for r in range(rounds2):
-            frame_all()
+            frame_all(r % 2 == 0)  # the first group regrows the ring after staging the small frames
             b.step(1)
-        frame_all()  # the final states, compared with downloads below
+        frame_all(True)  # the final states, compared with downloads below
         print(f"case batch frames: {len(taken)} frames, {off / 2**20:.0f} MiB", flush=True)
         b.frames_done()
         b.sync()
