@@ -2,10 +2,12 @@
 // kernel per model over device work-lists of tiles from every live grid, per-grid time-loop state
 // on the device, and a HIP graph per step.
 //
-// Buffer roles: at a step boundary each grid's state is in buf[0]; the first sweep of a step
-// reads buf[0] and writes buf[1], the second reads buf[1] and writes buf[0]. A grid whose loop
-// has ended is skipped by both sweeps, so its state stays in buf[0]. The sweep axis of each
-// phase follows the grid's splitting (alternating: x then y on even steps, y then x on odd).
+// Buffer roles: at a step boundary a grid's state is in buf[sw] (sw = 0 unless a Strang step,
+// three sweeps, has flipped it); the sweeps of a step alternate between the two buffers from
+// there. A grid whose loop has ended is skipped by every sweep, so its state stays put. The sweep
+// axis of each phase follows the grid's splitting (alternating: x then y on even steps, y then x
+// on odd; Strang: x, y, x). Readers of the state on the stream (frames, checksums) select
+// buf[sw] on the device when they run.
 #include "cfdart_batch.h"
 #include "cfdart_host.h"
 
@@ -49,6 +51,7 @@ __global__ __launch_bounds__(64) void batch_finalize_kernel(BatchGrid* grids, in
     if (fabs(t - c->T) < 1e-12) t = c->T;
     c->t = t;
     c->n = n;
+    if (bg->splitting == CFDART_STRANG) bg->sw ^= 1;  // three sweeps: the state moved to the other buffer
     bg->steps_left -= 1;
     c->active = (t < c->T) && bg->steps_left > 0 && !c->error;
     if (c->active) c->dt = dt_from_maxima(c, mu, mv, n, t, bg->dx, bg->dy);
@@ -66,6 +69,22 @@ __global__ void batch_mark_dead_kernel(BatchGrid* bg)
 __global__ void batch_zero_kernel(unsigned long long* p)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
+}
+
+// Frame staging: a grid's interior planes from buf[sw] (read when the kernel runs) into dst,
+// packed [component][y][x]; each workgroup copies whole rows, grid-stride over the rows.
+__global__ __launch_bounds__(256) void batch_stage_frame_kernel(const BatchGrid* bg, double* dst)
+{
+    const int nx = bg->nx, ny = bg->ny, nc = bg->model == ALLAIRE ? 6 : 4;
+    const long long plane = bg->plane, pitch = bg->pitch;
+    const double* src = bg->buf[bg->sw & 1];
+    const long long rows = (long long)ny * nc;
+    for (long long r = blockIdx.x; r < rows; r += gridDim.x) {
+        const long long k = r / ny, y = r - k * ny;
+        const double* a = src + k * plane + y * pitch;
+        double* d = dst + r * nx;
+        for (int x = threadIdx.x; x < nx; x += blockDim.x) d[x] = a[x];
+    }
 }
 
 __global__ void batch_set_lengths_kernel(BatchQueue* q, BatchQueue lengths)
@@ -156,8 +175,8 @@ struct cfdart_batch {
     bool have_graph;
     int graph_models;           // bit m: the captured steps launch model m's sweeps
     long long gstep;            // batch steps launched since creation
-    std::vector<int> cls;       // per slot: 0 / 1 alternating grid admitted at an even / odd step, 2 naive
-    int nclass[3];              // live grids per class
+    std::vector<int> cls;       // per slot: 0 / 1 alternating grid admitted at an even / odd step, 2 naive, 3 Strang
+    int nclass[4];              // live grids per class
     int arith;  // CFDART_ARITH_EXACT or CFDART_ARITH_FAST, for every grid of the batch
     int ntiles;
     std::multimap<size_t, double*> pool;  // freed state buffers for reuse, by size
@@ -372,6 +391,7 @@ int capture_graph(cfdart_batch* b)
 // only the sweeps that have work (a grid that has finished exits at once whatever the shape).
 int step_shape(const cfdart_batch* b, long long s)
 {
+    if (b->nclass[3] > 0) return cfd::STEP_XYX;  // a Strang grid: x, y, x (the others take two of the three)
     const bool x_first = b->nclass[2] > 0 || (b->nclass[0] > 0 && (s & 1) == 0) || (b->nclass[1] > 0 && (s & 1) == 1);
     const bool y_first = (b->nclass[0] > 0 && (s & 1) == 1) || (b->nclass[1] > 0 && (s & 1) == 0);
     if (x_first && y_first) return cfd::STEP_BOTH;
@@ -411,7 +431,7 @@ int cfdart_batch_create(int capacity, size_t mem_budget_bytes, cfdart_batch** ou
     b->graph_models = 0;
     b->gstep = 0;
     b->cls.assign(capacity, 0);
-    for (int k = 0; k < 3; ++k) b->nclass[k] = 0;
+    for (int k = 0; k < 4; ++k) b->nclass[k] = 0;
     b->arith = CFDART_ARITH_EXACT;
     b->ntiles = 0;
     b->d_status = nullptr;
@@ -519,8 +539,8 @@ int admit(cfdart_batch* b, const cfdart_desc* d, const double* host, const cfdar
           int* slot_out)
 {
     if (!b || !d || !slot_out) return bfail(CFDART_E_ARG, "null argument");
-    if (splitting != CFDART_NAIVE && splitting != CFDART_ALTERNATING)
-        return bfail(CFDART_E_ARG, "batched grids use naive or alternating splitting");
+    if (splitting != CFDART_NAIVE && splitting != CFDART_ALTERNATING && splitting != CFDART_STRANG)
+        return bfail(CFDART_E_ARG, "bad splitting scheme");
     if (max_steps < 1 || !(cfl > 0.0)) return bfail(CFDART_E_ARG, "need max_steps >= 1 and cfl > 0");
     // every check of cfdart_create, then the batch's own limits (MUSCL-Hancock, ghost width 2)
     const int rc = cfd::check_desc(d);
@@ -571,6 +591,7 @@ int admit(cfdart_batch* b, const cfdart_desc* d, const double* host, const cfdar
     g.xrows = xrows;
     g.steps_left = max_steps;
     g.live = 1;
+    g.sw = 0;
     std::memset(&g.ctrl, 0, sizeof(g.ctrl));
     g.ctrl.T = T;
     g.ctrl.cfl = cfl;
@@ -663,7 +684,7 @@ int admit(cfdart_batch* b, const cfdart_desc* d, const double* host, const cfdar
     }
     b->h_grids[slot] = g;
     b->live[slot] = 1;
-    b->cls[slot] = splitting == CFDART_ALTERNATING ? (int)(b->gstep & 1) : 2;
+    b->cls[slot] = splitting == CFDART_ALTERNATING ? (int)(b->gstep & 1) : (splitting == CFDART_STRANG ? 3 : 2);
     b->nclass[b->cls[slot]] += 1;
     b->bytes[slot] = need;
     b->used += need;
@@ -778,7 +799,9 @@ int cfdart_batch_download(cfdart_batch* b, int slot, double* host)
     const int nc = g.model == cfd::ALLAIRE ? 6 : 4, ng = 2, nx = g.nx, ny = g.ny, W = nx + 2 * ng;
     std::vector<double> soa((size_t)g.plane * nc);
     BCHK(hipStreamSynchronize(b->st));
-    BCHK(hipMemcpy(soa.data(), g.buf[0], sizeof(double) * soa.size(), hipMemcpyDeviceToHost));
+    int sw = 0;  // the buffer holding the state (flipped by Strang steps)
+    BCHK(hipMemcpy(&sw, &b->d_grids[slot].sw, sizeof(int), hipMemcpyDeviceToHost));
+    BCHK(hipMemcpy(soa.data(), g.buf[sw & 1], sizeof(double) * soa.size(), hipMemcpyDeviceToHost));
     auto at = [&](int i, int j, int k) -> double& { return host[((size_t)i * W + j) * nc + k]; };
     for (int y = 0; y < ny; ++y)
         for (int x = 0; x < nx; ++x)
@@ -923,10 +946,12 @@ int cfdart_batch_frames(cfdart_batch* b, int n, const int* slots, double* const*
                 ++it;
             }
         }
-        // stage the interior planes (contiguous, pitch removed) on the batch stream: the grid's
-        // state after a whole step is buf[0]
-        BCHK(hipMemcpy2DAsync(b->d_ring + off, sizeof(double) * g.nx, g.buf[0], sizeof(double) * g.pitch,
-                              sizeof(double) * g.nx, (size_t)g.ny * nc, hipMemcpyDeviceToDevice, b->st));
+        // stage the interior planes (contiguous, pitch removed) on the batch stream, from the
+        // buffer that holds the grid's state when the copy runs
+        const long long rows = (long long)g.ny * nc;
+        hipLaunchKernelGGL(cfd::batch_stage_frame_kernel, dim3((unsigned)std::min<long long>(rows, 4096)), dim3(256), 0,
+                           b->st, b->d_grids + slots[i], b->d_ring + off);
+        BCHK(hipGetLastError());
         pend.push_back(Pending{off, len, host_dsts[i]});
         b->ring_head = off + len;
     }
@@ -970,7 +995,8 @@ int cfdart_batch_checksum(cfdart_batch* b, int slot, unsigned long long* out)
     if (!out) return bfail(CFDART_E_ARG, "null output");
     const BatchGrid& g = b->h_grids[slot];
     BCHK(hipMemsetAsync(b->d_sum, 0, sizeof(unsigned long long), b->st));
-    BCHK(cfd::launch_checksum(g.buf[0], g.pitch, g.plane, g.nx, g.ny, g.model == cfd::ALLAIRE ? 6 : 4, b->d_sum, b->st));
+    BCHK(cfd::launch_checksum_sel(b->d_grids[slot].buf, &b->d_grids[slot].sw, g.pitch, g.plane, g.nx, g.ny,
+                                  g.model == cfd::ALLAIRE ? 6 : 4, b->d_sum, b->st));
     BCHK(hipMemcpyAsync(out, b->d_sum, sizeof(unsigned long long), hipMemcpyDeviceToHost, b->st));
     BCHK(hipStreamSynchronize(b->st));
     return 0;
@@ -984,7 +1010,8 @@ int cfdart_batch_checksum_async(cfdart_batch* b, int slot, unsigned long long* h
     const BatchGrid& g = b->h_grids[slot];
     hipLaunchKernelGGL(cfd::batch_zero_kernel, dim3(1), dim3(64), 0, b->st, b->d_sum);
     BCHK(hipGetLastError());
-    BCHK(cfd::launch_checksum(g.buf[0], g.pitch, g.plane, g.nx, g.ny, g.model == cfd::ALLAIRE ? 6 : 4, b->d_sum, b->st));
+    BCHK(cfd::launch_checksum_sel(b->d_grids[slot].buf, &b->d_grids[slot].sw, g.pitch, g.plane, g.nx, g.ny,
+                                  g.model == cfd::ALLAIRE ? 6 : 4, b->d_sum, b->st));
     BCHK(hipMemcpyAsync(host_out, b->d_sum, sizeof(unsigned long long), hipMemcpyDeviceToHost, b->st));
     return 0;
 }

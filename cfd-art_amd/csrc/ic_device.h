@@ -27,6 +27,10 @@ hipError_t launch_ic_bubbles(double* buf, int pitch, long long plane, const cfda
 // Adds the state checksum of the interior planes to *out (which the caller zeroes).
 hipError_t launch_checksum(const double* buf, int pitch, long long plane, int nx, int ny, int nc,
                            unsigned long long* out, hipStream_t st);
+// The same checksum of bufs[*sel & 1], the buffer chosen on the device when the kernel runs
+// (bufs and sel in device memory: a batch grid whose state buffer changes from step to step).
+hipError_t launch_checksum_sel(const double* const* bufs, const int* sel, int pitch, long long plane, int nx, int ny,
+                               int nc, unsigned long long* out, hipStream_t st);
 
 // Bytes of device memory generate_state needs for the draws.
 constexpr size_t IC_DRAW_BYTES = sizeof(Disc) * IC_MAX_DISCS;

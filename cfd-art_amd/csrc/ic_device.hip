@@ -84,8 +84,8 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long z)
 // Sum (mod 2^64) over every interior value of mix64(bits ^ (index * golden)), index = the
 // value's position in the packed [component][y][x] order: independent of the pitch and of
 // the order of summation, so a grid alone and the same grid in a batch compare directly.
-__global__ __launch_bounds__(256) void checksum_kernel(const double* buf, int pitch, long long plane, int nx, int ny,
-                                                       int nc, unsigned long long* out)
+__device__ __forceinline__ void checksum_body(const double* buf, int pitch, long long plane, int nx, int ny, int nc,
+                                              unsigned long long* out)
 {
     const long long per = (long long)nx * ny, total = per * nc;
     unsigned long long h = 0;
@@ -104,6 +104,20 @@ __global__ __launch_bounds__(256) void checksum_kernel(const double* buf, int pi
         __syncthreads();
     }
     if (threadIdx.x == 0) atomicAdd(out, s[0]);
+}
+
+__global__ __launch_bounds__(256) void checksum_kernel(const double* buf, int pitch, long long plane, int nx, int ny,
+                                                       int nc, unsigned long long* out)
+{
+    checksum_body(buf, pitch, plane, nx, ny, nc, out);
+}
+
+// The same over bufs[*sel], chosen on the device when the kernel runs.
+__global__ __launch_bounds__(256) void checksum_sel_kernel(const double* const* bufs, const int* sel, int pitch,
+                                                           long long plane, int nx, int ny, int nc,
+                                                           unsigned long long* out)
+{
+    checksum_body(bufs[*sel & 1], pitch, plane, nx, ny, nc, out);
 }
 
 int blocks_for(long long items)
@@ -145,6 +159,14 @@ hipError_t launch_checksum(const double* buf, int pitch, long long plane, int nx
 {
     hipLaunchKernelGGL(checksum_kernel, dim3(blocks_for((long long)nx * ny * nc)), dim3(256), 0, st, buf, pitch,
                        plane, nx, ny, nc, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_checksum_sel(const double* const* bufs, const int* sel, int pitch, long long plane, int nx, int ny,
+                               int nc, unsigned long long* out, hipStream_t st)
+{
+    hipLaunchKernelGGL(checksum_sel_kernel, dim3(blocks_for((long long)nx * ny * nc)), dim3(256), 0, st, bufs, sel,
+                       pitch, plane, nx, ny, nc, out);
     return hipGetLastError();
 }
 

@@ -603,8 +603,11 @@ __device__ __forceinline__ void y_tile(const GridView& g, double dt, int bx, int
 // the columns -- lanes along y, neighbours by DPP -- and overwrites its columns in LDS with the
 // results; the workgroup (256 threads) then stores the middle 60 rows. Same values as y_tile
 // (same cell and face stages, rotated frame).
+// last_here (LAST only): this sweep ends the grid's step, so the wave speeds are taken; a batch
+// launch of LAST sweeps can hold grids whose step ends in a later sweep.
 template <int M, bool LAST, int COLS, bool F = false>
-__device__ __forceinline__ void y_tile_t(const GridView& g, double dt, int x0, int y0, double* sU, Tally& tl)
+__device__ __forceinline__ void y_tile_t(const GridView& g, double dt, int x0, int y0, double* sU, Tally& tl,
+                                         bool last_here = true)
 {
     constexpr int NC = Model<M>::NC;
     constexpr int MX = Model<M>::MN, MY = Model<M>::MT;
@@ -697,7 +700,7 @@ __device__ __forceinline__ void y_tile_t(const GridView& g, double dt, int x0, i
                         Un[k] = sU[(k * 64 + rr) * PAD + col];
                         store_state(Un[k], out + k * g.plane);
                     }
-                    if (LAST) {
+                    if (LAST && last_here) {
                         double su, sv;
                         wave_speeds<F>(g.mat, Un, su, sv, (Model<M>*)nullptr);
                         tl.speeds<F>(su, sv);

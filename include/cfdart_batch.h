@@ -12,7 +12,9 @@
  * device-resident tables; the graph is re-captured only when a work-list outgrows its capacity or
  * the first grid of a model joins (or the last one leaves) the batch.
  *
- * Splitting schemes: CFDART_NAIVE and CFDART_ALTERNATING (two sweeps per step).
+ * Splitting schemes: CFDART_NAIVE, CFDART_ALTERNATING (two sweeps per step) and CFDART_STRANG
+ * (x(dt/2), y(dt), x(dt/2)); while the batch holds a Strang grid, its steps launch three sweep
+ * phases (x, y, x) and every other grid takes two of them.
  * Admissions, removals and tile changes take effect at the next cfdart_batch_step, which
  * rebuilds the work-list once for all of them.
  * Conventions as in cfdart.h: 0 or a negative CFDART_E_* code; one host thread per batch.

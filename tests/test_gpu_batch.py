@@ -36,7 +36,7 @@ def alone(d, h, cfl, steps, splitting=cfdart.ALTERNATING):
     return g.download(), t, n
 
 
-@pytest.mark.parametrize("splitting", [cfdart.ALTERNATING, cfdart.NAIVE])
+@pytest.mark.parametrize("splitting", [cfdart.ALTERNATING, cfdart.NAIVE, cfdart.STRANG])
 def test_batch_equals_alone(gpu, splitting):
     cases = mixed_cases(1)
     b = cfdart.Batch(16)
@@ -251,4 +251,57 @@ def test_naive_and_alternating_grids_together(gpu):
         print(f"case naive+alternating slot={s} splitting={sp} {d.nx}x{d.ny} steps={steps}", flush=True)
         assert nb == n == steps and tb == t and not act
         assert bitwise_equal(interior(b.download(s)), interior(ref))
+    b.close()
+
+
+def test_strang_grids_among_the_others(gpu):
+    """Strang grids (x(dt/2), y(dt), x(dt/2): three sweeps, so their state changes buffer at every
+    step) in a batch with naive and alternating grids, one alternating grid admitted at an odd
+    batch step: every grid ends bitwise where it ends alone, and a frame and a checksum queued
+    after an odd and after an even number of a Strang grid's steps hold its state at that point."""
+    import ctypes as C
+    L = cfdart.lib()
+    cases = mixed_cases(6, 5)
+    splits = [cfdart.STRANG, cfdart.NAIVE, cfdart.STRANG, cfdart.ALTERNATING, cfdart.STRANG]
+    runs = [(d, h, cfl, max(steps, 6)) for d, h, cfl, steps in cases]
+    b = cfdart.Batch(8)
+    slots = [b.add(d, h, splitting=sp, max_steps=n, cfl=cfl) for (d, h, cfl, n), sp in zip(runs[:3], splits[:3])]
+    d0, h0, c0, _ = runs[0]
+    nb = 8 * d0.nx * d0.ny * (6 if d0.model == 1 else 4)
+    host = L.cfdart_host_alloc(2 * nb + 16)
+    assert host
+    try:
+        b.step(3)                                   # Strang grid 0 has taken 3 steps: state in its second buffer
+        b.frame(slots[0], host)
+        cfdart._lib._check(L.cfdart_batch_checksum_async(b.h, slots[0], C.c_void_p(host + 2 * nb)))
+        for (d, h, cfl, n), sp in zip(runs[3:], splits[3:]):   # admitted at batch step 3
+            slots.append(b.add(d, h, splitting=sp, max_steps=n, cfl=cfl))
+        b.step(1)                                   # 4 steps: back in its first buffer
+        assert b.frames([slots[0]], [host + nb]) == [0]
+        cfdart._lib._check(L.cfdart_batch_checksum_async(b.h, slots[0], C.c_void_p(host + 2 * nb + 8)))
+        b.step(40)
+        b.frames_done()
+        b.sync()
+        nc = 6 if d0.model == 1 else 4
+        frames = [np.ctypeslib.as_array((C.c_double * (nb // 8)).from_address(host + o)).reshape(nc, d0.ny, d0.nx).copy()
+                  for o in (0, nb)]
+        sums = [C.c_ulonglong.from_address(host + 2 * nb + 8 * i).value for i in range(2)]
+    finally:
+        L.cfdart_host_free(C.c_void_p(host))
+    g = cfdart.Grid(d0)
+    g.upload(h0)
+    for k, (n, fr, cs) in enumerate(zip((3, 1), frames, sums)):
+        g.run(cfdart.STRANG, n, cfl=c0)
+        st = g.download()
+        print(f"case strang mid-run slot={slots[0]} after {3 + k} steps", flush=True)
+        assert cs == g.checksum()
+        assert np.array_equal(fr.view(np.uint64), np.moveaxis(st[2:-2, 2:-2, :], 2, 0).view(np.uint64))
+    g.close()
+    for (d, h, cfl, n), sp, s in zip(runs, splits, slots):
+        ref, t, nn = alone(d, h, cfl, n, sp)
+        tb, nb_, act = b.status(s)
+        print(f"case strang batch slot={s} splitting={sp} model={d.model} {d.nx}x{d.ny} steps={n}", flush=True)
+        assert nb_ == nn == n and tb == t and not act
+        assert bitwise_equal(interior(b.download(s)), interior(ref))
+        assert b.checksum(s) == cfdart.state_checksum(ref)
     b.close()
