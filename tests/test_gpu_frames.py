@@ -188,3 +188,40 @@ def test_batch_frames_match_state(gpu):
         L.cfdart_host_free(C.c_void_p(host))
         L.cfdart_host_free(C.c_void_p(sums))
     b.close()
+
+
+def test_batch_frames_skip_what_the_budget_cannot_stage(gpu):
+    """A frame whose staging would grow the device ring past the batch's memory budget is skipped
+    in a group (status CFDART_E_NOMEM) while the group's other frames are taken; alone it is
+    refused with that code."""
+    L = cfdart.lib()
+    ds, hs = cfdart.configs.euler_discs(200, 4, (0.05, 0.15), (0, 0, 0, 0), seed=61)
+    dl, hl = cfdart.configs.allaire_bubbles(1190, 3, seed=62)      # a frame of 8.5 M doubles: the ring must grow
+    def state(d):  # a grid's two state buffers as the batch counts them (256-byte rows, 2 MiB granules)
+        one = 8 * ((d.nx + 31) // 32 * 32) * d.ny * (6 if d.model == 1 else 4)
+        return 2 * (-(-one // (2 << 20)) * (2 << 20))
+    budget = 2 * state(ds) + state(dl) + 8 * 2**24 + (4 << 20)       # states, the first 128 MiB ring, some slack
+    b = cfdart.Batch(4, mem_budget_bytes=budget)
+    s0 = b.add(ds, hs, max_steps=10, cfl=0.8)
+    s1 = b.add(dl, hl, max_steps=10, cfl=0.5)
+    s2 = b.add(ds, hs, max_steps=10, cfl=0.8)
+    b.step(2)
+    nb = 8 * ds.nx * ds.ny * 4
+    host = L.cfdart_host_alloc(3 * nb)
+    assert host
+    try:
+        b.frame(s0, host + 2 * nb)                                     # a small frame: the 128 MiB ring
+        st = b.frames([s0, s1, s2], [host, host, host + nb])
+        print(f"case frames under budget: status {st}", flush=True)
+        assert st == [0, -2, 0]
+        with pytest.raises(cfdart.CfdartError) as e:
+            b.frame(s1, host)
+        assert e.value.code == -2
+        b.frames_done()
+        ref = np.moveaxis(b.download(s0)[2:-2, 2:-2, :], 2, 0)
+        for o in (0, nb, 2 * nb):
+            fr = np.ctypeslib.as_array((C.c_double * (nb // 8)).from_address(host + o)).reshape(4, ds.ny, ds.nx)
+            assert np.array_equal(fr.view(np.uint64), ref.view(np.uint64))
+    finally:
+        L.cfdart_host_free(C.c_void_p(host))
+    b.close()
