@@ -5,10 +5,11 @@
  * (euler_equations/euler_simulation_code/sourcecode/main.cpp:17-49). Each grid in a batch
  * follows exactly the reference's time loop (simulation.hpp:48-61) with its own CFL time step,
  * step budget and end time; its results are bitwise those of the same grid run alone through
- * cfdart.h in the same arithmetic (cfdart_batch_set_arith). Per sweep phase, one x-sweep and one y-sweep launch per model process the tiles of
- * every live grid from device work-lists (one tile per workgroup; a grid's tiles for the axis it
- * does not sweep in that phase exit at once), and the step (two phases, loop bookkeeping) is
- * captured once in a HIP graph and replayed. Grids join and leave between steps by editing
+ * cfdart.h in the same arithmetic (cfdart_batch_set_arith). Per sweep phase, one x-sweep and
+ * one y-sweep launch per model process the tiles of every live grid from device work-lists (one
+ * tile per workgroup; a grid's tiles for the axis it does not sweep in that phase exit at once),
+ * and the step (two or three phases, loop bookkeeping) is captured once in a HIP graph and
+ * replayed. Grids join and leave between steps by editing
  * device-resident tables; the graph is re-captured only when a work-list outgrows its capacity or
  * the first grid of a model joins (or the last one leaves) the batch.
  *
